@@ -1,0 +1,627 @@
+// cgemm3m_host.cpp -- aclblasCgemm3m: validation, planner, workspace carving, launches.
+//
+// Reference: entry ops-blas/blas/gemm/arch35/cgemm3m_host.cpp:17-23 and the shared
+// dispatcher cgemm_host.cpp (validation :589-634, :687-699; quick returns :636-663,
+// :701-730; planner :130-285; workspace :287-360, :665-685; launches :362-551).
+// The planner here is sized for MI355X (256 CUs, 160 KiB LDS, 288 GB HBM) and is a pure
+// function of (shape, scalars, op, CU count, config), so the same call always takes the
+// same path and reruns are byte-identical.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "host_internal.h"
+#include "launch.h"
+
+namespace c3m {
+
+static const char* kOp = "aclblasCgemm3m";
+
+StageTimer::StageTimer(aclblasHandle_t h, int stage) : h_(h), stage_(stage)
+{
+    if (h_->timing && hipEventCreate(&start_) == hipSuccess) {
+        (void)hipEventRecord(start_, h_->stream);
+    } else {
+        start_ = nullptr;
+    }
+}
+
+StageTimer::~StageTimer()
+{
+    if (start_ == nullptr) return;
+    hipEvent_t stop = nullptr;
+    if (hipEventCreate(&stop) == hipSuccess && hipEventRecord(stop, h_->stream) == hipSuccess) {
+        h_->timed.push_back({stage_, start_, stop});
+    } else {
+        (void)hipEventDestroy(start_);
+        if (stop != nullptr) (void)hipEventDestroy(stop);
+    }
+}
+
+static bool is_op(aclblasOperation_t t) { return t == ACLBLAS_OP_N || t == ACLBLAS_OP_T || t == ACLBLAS_OP_C; }
+static bool is_zero(const aclblasComplex& z) { return z.real == 0.0f && z.imag == 0.0f; }
+static bool is_one(const aclblasComplex& z) { return z.real == 1.0f && z.imag == 0.0f; }
+
+// Same checks, order and status codes as ValidateCgemmParams (cgemm_host.cpp:589-634),
+// with the 3m remap of INVALID_ENUM to INVALID_VALUE (:711-714).
+aclblasStatus_t validate(aclblasHandle_t h, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n, int k,
+                         const aclblasComplex* alpha, int lda, int ldb, const aclblasComplex* beta, int ldc,
+                         const char* op)
+{
+    if (h == nullptr) {
+        log_error(op, "handle is nullptr");
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    if (!is_op(ta)) {
+        log_error(op, "invalid transa: %d", (int)ta);
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (!is_op(tb)) {
+        log_error(op, "invalid transb: %d", (int)tb);
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (m < 0 || n < 0 || k < 0) {
+        log_error(op, "m, n, k must be non-negative: m=%d, n=%d, k=%d", m, n, k);
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    const int ldaMin = ta != ACLBLAS_OP_N ? std::max(1, k) : std::max(1, m);
+    if (lda < ldaMin) {
+        log_error(op, "lda=%d must be >= %d", lda, ldaMin);
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    const int ldbMin = tb != ACLBLAS_OP_N ? std::max(1, n) : std::max(1, k);
+    if (ldb < ldbMin) {
+        log_error(op, "ldb=%d must be >= %d", ldb, ldbMin);
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (ldc < std::max(1, m)) {
+        log_error(op, "ldc=%d must be >= %d", ldc, std::max(1, m));
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (alpha == nullptr) {
+        log_error(op, "alpha must not be nullptr");
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (beta == nullptr) {
+        log_error(op, "beta must not be nullptr");
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+// ValidateCgemmPointers (cgemm_host.cpp:687-699), called with k = 0 when alpha == 0.
+static aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op)
+{
+    if (C == nullptr) {
+        log_error(op, "C must not be nullptr for a non-empty output");
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (k > 0 && (A == nullptr || B == nullptr)) {
+        log_error(op, "A and B must not be nullptr when k > 0");
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+static size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+static bool mul_ok(size_t a, size_t b, size_t& out)
+{
+    if (a != 0 && b > std::numeric_limits<size_t>::max() / a) return false;
+    out = a * b;
+    return true;
+}
+
+static bool add_region(size_t bytes, size_t& total, size_t& off)
+{
+    if (total > std::numeric_limits<size_t>::max() - kAlign) return false;
+    off = round_up(total, kAlign);
+    if (bytes > std::numeric_limits<size_t>::max() - off) return false;
+    total = off + bytes;
+    return true;
+}
+
+bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan& pl, const char* op)
+{
+    if (cus <= 0) cus = 256;
+    pl = Plan();
+    pl.precision = cfg ? cfg->precision : ACLBLAS_CGEMM3M_FP32;
+    if (p.m == 0 || p.n == 0) {
+        pl.path = 0;
+        return true;
+    }
+    if (p.k == 0 || is_zero(p.alpha)) {
+        pl.path = is_one(p.beta) ? 0 : 3;
+        return true;
+    }
+    double unit = 0x1p-24;  // unit roundoff of the product arithmetic (correction screen)
+    switch (pl.precision) {
+        case ACLBLAS_CGEMM3M_FP32: pl.format = FMT_F32; pl.pieces = 1; pl.bk = 16; break;
+        case ACLBLAS_CGEMM3M_BF16X3: pl.format = FMT_BF16; pl.pieces = 2; pl.bk = 32; unit = 0x1p-16; break;
+        case ACLBLAS_CGEMM3M_BF16X6: pl.format = FMT_BF16; pl.pieces = 3; pl.bk = 32; break;
+        case ACLBLAS_CGEMM3M_FP16_SCALED: pl.format = FMT_F16; pl.pieces = 1; pl.bk = 32; unit = 0x1p-11; break;
+        default: log_error(op, "invalid precision %d", pl.precision); return false;
+    }
+    (void)unit;
+    pl.identity = (cfg && cfg->identity == ID_STANDARD) ? ID_STANDARD : ID_FUSED;
+    if (cfg && cfg->identity != 0 && cfg->identity != ID_FUSED && cfg->identity != ID_STANDARD) {
+        log_error(op, "invalid identity %d", cfg->identity);
+        return false;
+    }
+    const long long m = p.m, n = p.n, k = p.k;
+    auto tiles_for = [&](int t) { return ((m + t - 1) / t) * ((n + t - 1) / t); };
+    int tile = 0;
+    if (cfg && cfg->tile != 0) {
+        if (cfg->tile != 64 && cfg->tile != 128) {
+            log_error(op, "invalid tile %d", cfg->tile);
+            return false;
+        }
+        tile = cfg->tile;
+    } else {
+        tile = tiles_for(128) >= cus ? 128 : 64;
+    }
+    pl.bm = pl.bn = tile;
+    pl.mPad = (int)round_up((size_t)m, 128);
+    pl.nPad = (int)round_up((size_t)n, 128);
+    pl.kPad = (int)round_up((size_t)k, 32);
+    pl.kSteps = pl.kPad / pl.bk;
+    const long long tiles = tiles_for(tile);
+    if (cfg && cfg->kSplit != 0) {
+        if (cfg->kSplit < 1 || cfg->kSplit > 16) {
+            log_error(op, "invalid kSplit %d", cfg->kSplit);
+            return false;
+        }
+        pl.parts = std::min(cfg->kSplit, pl.kSteps);
+    } else {
+        // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition.
+        pl.parts = 1;
+        if (tiles < cus) {
+            long long parts = (cus + tiles - 1) / tiles;
+            parts = std::min<long long>(parts, 16);
+            parts = std::min<long long>(parts, std::max(1, pl.kSteps * pl.bk / 256));
+            pl.parts = (int)std::max<long long>(parts, 1);
+        }
+    }
+    if (tiles * pl.parts > std::numeric_limits<int>::max()) {
+        log_error(op, "problem too large for one launch");
+        return false;
+    }
+    pl.path = pl.parts > 1 ? 2 : 1;
+    if (cfg && (cfg->correction < -1 || cfg->correction > 1)) {
+        log_error(op, "invalid correction flag %d", cfg->correction);
+        return false;
+    }
+    if (cfg && cfg->correction != 0)
+        pl.correction = cfg->correction > 0;
+    else
+        pl.correction = pl.precision == ACLBLAS_CGEMM3M_FP32 && k >= 1024;
+
+    // Workspace regions.
+    const size_t elem = pl.format == FMT_F32 ? 4 : 2;
+    size_t t1, t2;
+    if (!mul_ok((size_t)pl.mPad, (size_t)pl.kPad, t1) || !mul_ok(t1, elem, pl.aPlaneBytes)) return false;
+    if (!mul_ok((size_t)pl.nPad, (size_t)pl.kPad, t2) || !mul_ok(t2, elem, pl.bPlaneBytes)) return false;
+    pl.planesA = pl.planesB = 3 * pl.pieces;
+    size_t total = 0, bytes = 0;
+    if (!mul_ok(pl.aPlaneBytes, (size_t)pl.planesA, bytes) || !add_region(bytes, total, pl.offA)) return false;
+    if (!mul_ok(pl.bPlaneBytes, (size_t)pl.planesB, bytes) || !add_region(bytes, total, pl.offB)) return false;
+    if (pl.parts > 1) {
+        size_t mn;
+        if (!mul_ok((size_t)pl.mPad, (size_t)pl.nPad, mn)) return false;
+        pl.partStride = 2 * mn;
+        if (!mul_ok(pl.partStride * 4, (size_t)pl.parts, bytes) || !add_region(bytes, total, pl.offPartial))
+            return false;
+    }
+    pl.kTiles = (pl.kPad + 63) / 64;
+    if (pl.correction) {
+        const size_t outs = (size_t)m * (size_t)n;
+        pl.bitmapWords = (long long)((outs + 63) / 64);
+        if (!add_region((size_t)pl.kTiles * pl.mPad * 8, total, pl.offEPA)) return false;
+        if (!add_region((size_t)pl.kTiles * pl.nPad * 8, total, pl.offEPB)) return false;
+        if (!add_region((size_t)pl.mPad * 8, total, pl.offEA)) return false;
+        if (!add_region((size_t)pl.nPad * 8, total, pl.offEB)) return false;
+        if (!add_region((size_t)pl.bitmapWords * 8, total, pl.offBitmap)) return false;
+        if (!is_zero(p.beta) && !add_region(outs * 8, total, pl.offShadow)) return false;
+        if (!add_region(256, total, pl.offCounter)) return false;
+    }
+    if (pl.format == FMT_F16) {
+        const size_t kb = (size_t)(pl.kPad + 127) / 128;
+        if (!add_region(3 * (size_t)(pl.mPad / 128) * kb * 4, total, pl.offScaleA)) return false;
+        if (!add_region(3 * (size_t)(pl.nPad / 128) * kb * 4, total, pl.offScaleB)) return false;
+    }
+    pl.total = total;
+    return true;
+}
+
+static PackArgs pack_args(const Problem& p, const Plan& pl, char* ws, int role)
+{
+    PackArgs a{};
+    const bool isA = role == 0;
+    const aclblasOperation_t t = isA ? p.ta : p.tb;
+    a.src = isA ? (const void*)p.A : (const void*)p.B;
+    a.ld = isA ? p.lda : p.ldb;
+    a.mn = isA ? p.m : p.n;
+    a.kk = p.k;
+    a.mnPad = isA ? pl.mPad : pl.nPad;
+    a.kPad = pl.kPad;
+    // op(A)(i,l): N reads A[i + l*lda] (i adjacent); op(B)(l,j): T/C reads B[j + l*ldb] (j adjacent).
+    a.mnContig = isA ? (t == ACLBLAS_OP_N) : (t != ACLBLAS_OP_N);
+    a.conj = t == ACLBLAS_OP_C;
+    a.role = role;
+    a.identity = pl.identity;
+    a.format = pl.format;
+    a.pieces = pl.pieces;
+    a.planes = ws + (isA ? pl.offA : pl.offB);
+    a.planeBytes = isA ? pl.aPlaneBytes : pl.bPlaneBytes;
+    a.energyPartial = pl.correction ? reinterpret_cast<double*>(ws + (isA ? pl.offEPA : pl.offEPB)) : nullptr;
+    a.scaleExp = pl.format == FMT_F16 ? reinterpret_cast<const int*>(ws + (isA ? pl.offScaleA : pl.offScaleB))
+                                      : nullptr;
+    return a;
+}
+
+aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, char* ws, const char* op)
+{
+    hipStream_t s = h->stream;
+    hipError_t e = hipSuccess;
+    h->lastCounter = nullptr;
+    if (pl.path == 0) return ACLBLAS_STATUS_SUCCESS;
+    if (pl.path == 3) {
+        ScaleCArgs a{p.m, p.n, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc};
+        StageTimer t(h, ST_SCALE);
+        e = launch_scale_c(a, s);
+    } else {
+        const PackArgs pa = pack_args(p, pl, ws, 0);
+        const PackArgs pb = pack_args(p, pl, ws, 1);
+        if (pl.format == FMT_F16) {
+            StageTimer t(h, ST_PACK);
+            for (const PackArgs* x : {&pa, &pb}) {
+                ScaleArgs sa{x->src, x->ld, x->mn, x->kk, x->mnPad, x->kPad, x->mnContig, x->conj, x->role,
+                             x->identity, const_cast<int*>(x->scaleExp)};
+                if (e == hipSuccess) e = launch_scale_exponents(sa, s);
+            }
+        }
+        if (e == hipSuccess) {
+            StageTimer t(h, ST_PACK);
+            e = launch_pack_pair(pa, pb, s);
+        }
+        double* eA = nullptr;
+        double* eB = nullptr;
+        unsigned long long* bitmap = nullptr;
+        unsigned int* counter = nullptr;
+        if (pl.correction) {
+            eA = reinterpret_cast<double*>(ws + pl.offEA);
+            eB = reinterpret_cast<double*>(ws + pl.offEB);
+            bitmap = reinterpret_cast<unsigned long long*>(ws + pl.offBitmap);
+            counter = reinterpret_cast<unsigned int*>(ws + pl.offCounter);
+            EnergyArgs ea{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles, eA, eB,
+                          bitmap, pl.bitmapWords, counter};
+            if (e == hipSuccess) {
+                StageTimer t(h, ST_ENERGY);
+                e = launch_energy(ea, s);
+            }
+        }
+        const double amag = std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
+        double unit = 0x1p-24;
+        if (pl.precision == ACLBLAS_CGEMM3M_BF16X3) unit = 0x1p-16;
+        if (pl.precision == ACLBLAS_CGEMM3M_FP16_SCALED) unit = 0x1p-11;
+        GemmArgs g{};
+        g.Ap = ws + pl.offA;
+        g.aPlaneBytes = pl.aPlaneBytes;
+        g.mPad = pl.mPad;
+        g.Bp = ws + pl.offB;
+        g.bPlaneBytes = pl.bPlaneBytes;
+        g.nPad = pl.nPad;
+        g.m = p.m;
+        g.n = p.n;
+        g.kSteps = pl.kSteps;
+        g.parts = pl.parts;
+        g.tilesM = (p.m + pl.bm - 1) / pl.bm;
+        g.tilesN = (p.n + pl.bn - 1) / pl.bn;
+        g.identity = pl.identity;
+        g.alphaR = p.alpha.real;
+        g.alphaI = p.alpha.imag;
+        g.betaR = p.beta.real;
+        g.betaI = p.beta.imag;
+        g.betaZero = is_zero(p.beta) ? 1 : 0;
+        g.C = p.C;
+        g.ldc = p.ldc;
+        g.partial = pl.parts > 1 ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr;
+        g.partStride = pl.partStride;
+        g.scaleA = pa.scaleExp;
+        g.scaleB = pb.scaleExp;
+        g.kBlocks128 = (pl.kPad + 127) / 128;
+        g.correction = pl.correction;
+        g.eA = eA;
+        g.eB = eB;
+        g.corrScale = amag * unit;
+        g.corrTau = 0x1p-8;
+        g.bitmap = bitmap;
+        g.shadow = pl.correction && !is_zero(p.beta) ? ws + pl.offShadow : nullptr;
+        if (e == hipSuccess) {
+            StageTimer t(h, ST_GEMM);
+            if (pl.format == FMT_F32)
+                e = launch_gemm_f32(g, pl.bm, pl.bn, s);
+            else if (pl.format == FMT_BF16)
+                e = launch_gemm_bf16(g, pl.bm, pl.bn, pl.pieces, s);
+            else
+                e = launch_gemm_f16(g, pl.bm, pl.bn, s);
+        }
+        if (pl.parts > 1 && e == hipSuccess) {
+            MergeArgs mg{};
+            mg.partial = g.partial;
+            mg.partStride = pl.partStride;
+            mg.parts = pl.parts;
+            mg.ldp = pl.mPad;
+            mg.m = p.m;
+            mg.n = p.n;
+            mg.alphaR = g.alphaR;
+            mg.alphaI = g.alphaI;
+            mg.betaR = g.betaR;
+            mg.betaI = g.betaI;
+            mg.betaZero = g.betaZero;
+            mg.C = p.C;
+            mg.ldc = p.ldc;
+            mg.correction = pl.correction;
+            mg.eA = eA;
+            mg.eB = eB;
+            mg.corrScale = g.corrScale;
+            mg.corrTau = g.corrTau;
+            mg.bitmap = bitmap;
+            mg.shadow = g.shadow;
+            StageTimer t(h, ST_MERGE);
+            e = launch_merge(mg, s);
+        }
+        if (pl.correction && e == hipSuccess) {
+            RecomputeArgs r{};
+            r.A = p.A;
+            r.lda = p.lda;
+            r.opA = (int)p.ta;
+            r.B = p.B;
+            r.ldb = p.ldb;
+            r.opB = (int)p.tb;
+            r.m = p.m;
+            r.n = p.n;
+            r.k = p.k;
+            r.alphaR = p.alpha.real;
+            r.alphaI = p.alpha.imag;
+            r.betaR = p.beta.real;
+            r.betaI = p.beta.imag;
+            r.betaZero = g.betaZero;
+            r.bitmap = bitmap;
+            r.words = pl.bitmapWords;
+            r.shadow = g.shadow;
+            r.C = p.C;
+            r.ldc = p.ldc;
+            r.counter = counter;
+            StageTimer t(h, ST_RECOMPUTE);
+            e = launch_recompute(r, s);
+            h->lastCounter = counter;
+        }
+    }
+    if (e != hipSuccess) {
+        log_error(op, "kernel launch failed: %s", hipGetErrorString(e));
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    if (h->timing) h->timedCalls++;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+static void record_plan(aclblasHandle_t h, const Plan& pl)
+{
+    h->lastPlan[0] = pl.path;
+    h->lastPlan[1] = pl.bm;
+    h->lastPlan[2] = pl.parts;
+    h->lastPlan[3] = pl.correction;
+    h->lastPlan[4] = pl.precision;
+    h->lastPlan[5] = pl.identity;
+    h->lastPlan[6] = -1;
+    h->lastPlan[7] = (long long)pl.total;
+}
+
+static aclblasStatus_t run(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
+                           int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                           const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
+                           int ldc, const aclblasCgemm3mConfig* cfg, const char* op)
+{
+    aclblasStatus_t st = validate(handle, ta, tb, m, n, k, alpha, lda, ldb, beta, ldc, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    if (m == 0 || n == 0) {
+        Plan none;
+        record_plan(handle, none);
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    const Problem p{ta, tb, m, n, k, *alpha, *beta, A, lda, B, ldb, C, ldc};
+    st = validate_pointers(is_zero(p.alpha) ? 0 : k, A, B, C, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    Plan pl;
+    if (!make_plan(p, cfg, handle->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
+    record_plan(handle, pl);
+    if (pl.path == 0) return ACLBLAS_STATUS_SUCCESS;
+    st = ensure_device(handle);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        log_error(op, "no HIP device available");
+        return st;
+    }
+    // The plan depends on the CU count; re-plan now that it is known.
+    if (!make_plan(p, cfg, handle->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
+    record_plan(handle, pl);
+    char* ws = nullptr;
+    if (pl.total > 0) {
+        st = acquire_workspace(handle, pl.total, &ws, op);
+        if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    }
+    return enqueue(handle, p, pl, ws, op);
+}
+
+}  // namespace c3m
+
+using namespace c3m;
+
+extern "C" aclblasStatus_t aclblasCgemm3m(aclblasHandle_t handle, aclblasOperation_t transa,
+                                          aclblasOperation_t transb, int m, int n, int k,
+                                          const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                                          const aclblasComplex* B, int ldb, const aclblasComplex* beta,
+                                          aclblasComplex* C, int ldc)
+{
+    return run(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, kOp);
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mEx(aclblasHandle_t handle, aclblasOperation_t transa,
+                                            aclblasOperation_t transb, int m, int n, int k,
+                                            const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                                            const aclblasComplex* B, int ldb, const aclblasComplex* beta,
+                                            aclblasComplex* C, int ldc, const aclblasCgemm3mConfig* config)
+{
+    return run(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, config, "aclblasCgemm3mEx");
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGetWorkspaceSize(aclblasHandle_t handle, aclblasOperation_t transa,
+                                                          aclblasOperation_t transb, int m, int n, int k,
+                                                          const aclblasComplex* alpha, const aclblasComplex* beta,
+                                                          const aclblasCgemm3mConfig* config, size_t* bytes)
+{
+    const char* op = "aclblasCgemm3mGetWorkspaceSize";
+    if (bytes == nullptr) return ACLBLAS_STATUS_INVALID_VALUE;
+    const int lda = std::max(1, transa != ACLBLAS_OP_N ? k : m);
+    const int ldb = std::max(1, transb != ACLBLAS_OP_N ? n : k);
+    aclblasStatus_t st = validate(handle, transa, transb, m, n, k, alpha, lda, ldb, beta, std::max(1, m), op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    const Problem p{transa, transb, m, n, k, *alpha, *beta, nullptr, lda, nullptr, ldb, nullptr, std::max(1, m)};
+    if (ensure_device(handle) != ACLBLAS_STATUS_SUCCESS) handle->cus = 0;
+    Plan pl;
+    if (!make_plan(p, config, handle->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
+    *bytes = pl.total;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGetLastPlan(aclblasHandle_t handle, int sync, long long* info)
+{
+    if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    if (info == nullptr) return ACLBLAS_STATUS_INVALID_VALUE;
+    for (int t = 0; t < 8; ++t) info[t] = handle->lastPlan[t];
+    info[6] = -1;
+    if (sync && handle->lastCounter != nullptr) {
+        unsigned int v = 0;
+        if (hipStreamSynchronize(handle->stream) != hipSuccess ||
+            hipMemcpy(&v, handle->lastCounter, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) {
+            return ACLBLAS_STATUS_EXECUTION_FAILED;
+        }
+        info[6] = v;
+    }
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mSetTiming(aclblasHandle_t handle, int enable)
+{
+    if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    handle->timing = enable ? 1 : 0;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGetTiming(aclblasHandle_t handle, double* ms, long long* calls)
+{
+    if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    if (ms == nullptr || calls == nullptr) return ACLBLAS_STATUS_INVALID_VALUE;
+    for (int t = 0; t < ST_COUNT; ++t) ms[t] = 0.0;
+    aclblasStatus_t st = ACLBLAS_STATUS_SUCCESS;
+    for (auto& ev : handle->timed) {
+        float v = 0.0f;
+        if (hipEventSynchronize(ev.stop) == hipSuccess && hipEventElapsedTime(&v, ev.start, ev.stop) == hipSuccess)
+            ms[ev.stage] += v;
+        else
+            st = ACLBLAS_STATUS_EXECUTION_FAILED;
+        (void)hipEventDestroy(ev.start);
+        (void)hipEventDestroy(ev.stop);
+    }
+    handle->timed.clear();
+    *calls = handle->timedCalls;
+    handle->timedCalls = 0;
+    return st;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mResidual(aclblasHandle_t handle, aclblasOperation_t transa,
+                                                  aclblasOperation_t transb, int m, int n, int k,
+                                                  const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                                                  const aclblasComplex* B, int ldb, const aclblasComplex* beta,
+                                                  const aclblasComplex* Cinit, const aclblasComplex* C, int ldc,
+                                                  double* out)
+{
+    const char* op = "aclblasCgemm3mResidual";
+    aclblasStatus_t st = validate(handle, transa, transb, m, n, k, alpha, lda, ldb, beta, ldc, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    if (out == nullptr || C == nullptr || (k > 0 && (A == nullptr || B == nullptr)) ||
+        (!is_zero(*beta) && Cinit == nullptr))
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    for (int t = 0; t < 4; ++t) out[t] = 0.0;
+    if (m == 0 || n == 0) return ACLBLAS_STATUS_SUCCESS;
+    st = ensure_device(handle);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    const int bm = (m + 63) / 64, bn = (n + 63) / 64;
+    const size_t nblk = (size_t)bm * bn;
+    double* dOut = nullptr;
+    if (hipMalloc(&dOut, nblk * 5 * sizeof(double)) != hipSuccess) return ACLBLAS_STATUS_ALLOC_FAILED;
+    ResidualArgs r{};
+    r.A = A;
+    r.lda = lda;
+    r.opA = (int)transa;
+    r.B = B;
+    r.ldb = ldb;
+    r.opB = (int)transb;
+    r.m = m;
+    r.n = n;
+    r.k = k;
+    r.alphaR = alpha->real;
+    r.alphaI = alpha->imag;
+    r.betaR = beta->real;
+    r.betaI = beta->imag;
+    r.betaZero = is_zero(*beta) ? 1 : 0;
+    r.Cinit = Cinit;
+    r.C = C;
+    r.ldc = ldc;
+    r.out = dOut;
+    hipError_t e = launch_residual(r, bm, bn, handle->stream);
+    std::vector<double> host(nblk * 5);
+    if (e == hipSuccess) e = hipStreamSynchronize(handle->stream);
+    if (e == hipSuccess) e = hipMemcpy(host.data(), dOut, host.size() * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dOut);
+    if (e != hipSuccess) {
+        log_error(op, "residual failed: %s", hipGetErrorString(e));
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    double mx = 0.0, sum = 0.0, cnt = 0.0, mxAbs = 0.0, mxRef = 0.0;
+    for (size_t b = 0; b < nblk; ++b) {
+        mx = std::max(mx, host[b * 5 + 0]);
+        sum += host[b * 5 + 1];
+        cnt += host[b * 5 + 2];
+        mxAbs = std::max(mxAbs, host[b * 5 + 3]);
+        mxRef = std::max(mxRef, host[b * 5 + 4]);
+    }
+    out[0] = mx;
+    out[1] = cnt > 0 ? sum / cnt : 0.0;
+    out[2] = mxRef > 0 ? mxAbs / mxRef : mxAbs;
+    out[3] = mxRef;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t, int, const aclblasOperation_t*,
+                                                        const aclblasOperation_t*, const int*, const int*,
+                                                        const int*, const aclblasComplex*,
+                                                        const aclblasComplex* const*, const int*,
+                                                        const aclblasComplex* const*, const int*,
+                                                        const aclblasComplex*, aclblasComplex* const*, const int*,
+                                                        int)
+{
+    return ACLBLAS_STATUS_NOT_SUPPORTED;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasHandle_t, int,
+                                                                        const aclblasOperation_t*,
+                                                                        const aclblasOperation_t*, const int*,
+                                                                        const int*, const int*, int, size_t*)
+{
+    return ACLBLAS_STATUS_NOT_SUPPORTED;
+}

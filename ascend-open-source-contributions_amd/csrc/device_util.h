@@ -1,0 +1,91 @@
+// device_util.h -- small device helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernel_args.h"
+
+namespace c3m {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// Round-to-nearest-even FP32 -> BF16 bits (NaN stays a quiet NaN).
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Logical output tile of hardware block `bid`. Workgroups are dealt round-robin to the 8
+// XCDs, each with its own L2; remapping gives every XCD one contiguous range of logical
+// tiles, and tiles are walked in groups of 8 tile rows so that the tiles in flight on one
+// XCD share A row-panels and B column-panels in that L2.
+__device__ __forceinline__ void map_tile(int bid, int nblocks, int tilesM, int tilesN, int& part, int& tm,
+                                         int& tn)
+{
+    int logical = bid;
+    if ((nblocks & 7) == 0) {
+        const int per = nblocks >> 3;
+        logical = (bid & 7) * per + (bid >> 3);
+    }
+    const int T = tilesM * tilesN;
+    part = logical / T;
+    const int t = logical - part * T;
+    constexpr int G = 8;
+    const int groupSize = G * tilesN;
+    const int g = t / groupSize;
+    const int first = g * G;
+    const int rows = min(tilesM - first, G);
+    const int w = t - g * groupSize;
+    tm = first + w % rows;
+    tn = w / rows;
+}
+
+// The three products of one output combined into (re, im).
+__device__ __forceinline__ void combine3(int identity, float p0, float p1, float p2, float& re, float& im)
+{
+    if (identity == ID_STANDARD) {
+        re = p0 - p1;
+        im = p2 - p0 - p1;
+    } else {
+        re = p0 - p2;
+        im = p0 + p1;
+    }
+}
+
+// alpha*(re, im) + beta*c0 in FP32, C never read when beta == 0.
+__device__ __forceinline__ float2 apply_scalars(float re, float im, float ar, float ai, float br, float bi,
+                                                bool betaZero, float2 c0)
+{
+    float2 c;
+    c.x = ar * re - ai * im;
+    c.y = ar * im + ai * re;
+    if (!betaZero) {
+        c.x += br * c0.x - bi * c0.y;
+        c.y += br * c0.y + bi * c0.x;
+    }
+    return c;
+}
+
+// Rounding-correction screen: flag an output whose FP32 error estimate
+// corrScale * |op(A)_i|_2 * |op(B)_j|_2 exceeds corrTau * |c|, or that is not finite.
+__device__ __forceinline__ bool needs_recompute(float2 c, double ea, double eb, double corrScale, double corrTau)
+{
+    if (!isfinite(c.x) || !isfinite(c.y)) return true;
+    const double mag = sqrt((double)c.x * c.x + (double)c.y * c.y);
+    const double est = corrScale * sqrt(ea * eb);
+    return !(est <= corrTau * mag);
+}
+
+__device__ __forceinline__ void flag_output(unsigned long long* bitmap, long long lin)
+{
+    atomicOr(&bitmap[lin >> 6], 1ull << (lin & 63));
+}
+
+}  // namespace c3m

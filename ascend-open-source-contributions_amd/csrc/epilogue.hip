@@ -1,0 +1,254 @@
+// epilogue.hip -- HBM-bound output kernels and the FP64 passes.
+//
+//   merge_kernel     K-split epilogue (reference K9, ops-blas/blas/gemm/arch35/
+//                    cgemm_epilogue_kernel.cpp:34-125, :231-332): partitions summed in index
+//                    order p = 0..P-1 (no atomics, so reruns are byte-identical), then
+//                    alpha*(re, im) + beta*C; C is not read when beta == 0; only rows < m
+//                    are written, so ldc padding is never touched.
+//   scale_c_kernel   alpha == 0 or k == 0 (reference HandleCgemmAlphaZero,
+//                    cgemm_host.cpp:636-663): C = beta*C, or zeros when beta == 0.
+//   recompute_kernel rounding correction (reference K5-K8, cgemm3m_rounding_kernel.cpp:
+//                    132-497, re-designed): every output flagged by the GEMM epilogue is
+//                    recomputed from the original complex64 inputs in FP64, each lane
+//                    taking a fixed residue class of k in increasing order and the 64 lane
+//                    sums combined by a fixed butterfly, then alpha/beta applied in FP64
+//                    and the result rounded once to FP32.
+//   residual_kernel  FP64 complex GEMM of the same inputs compared with the FP32 result
+//                    (error report for tests and benches only).
+#include "device_util.h"
+#include "launch.h"
+
+namespace c3m {
+
+__global__ __launch_bounds__(256) void merge_kernel(MergeArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.m) return;
+    for (int j = blockIdx.y; j < a.n; j += gridDim.y) {
+    const size_t o = (size_t)j * a.ldp + i;
+    float re = 0.0f, im = 0.0f;
+    const size_t plane = a.partStride;  // partition stride in floats
+    const size_t imOff = plane / 2;     // re plane then im plane
+    for (int p = 0; p < a.parts; ++p) {
+        re += a.partial[(size_t)p * plane + o];
+        im += a.partial[(size_t)p * plane + imOff + o];
+    }
+    float2* C = reinterpret_cast<float2*>(a.C);
+    const size_t co = (size_t)j * a.ldc + i;
+    const bool bz = a.betaZero != 0;
+    float2 c0 = make_float2(0.0f, 0.0f);
+    if (!bz) c0 = C[co];
+    const float2 c = apply_scalars(re, im, a.alphaR, a.alphaI, a.betaR, a.betaI, bz, c0);
+    if (a.correction && needs_recompute(c, a.eA[i], a.eB[j], a.corrScale, a.corrTau)) {
+        const long long lin = (long long)j * a.m + i;
+        flag_output(a.bitmap, lin);
+        if (!bz) reinterpret_cast<float2*>(a.shadow)[lin] = c0;
+    }
+    C[co] = c;
+    }
+}
+
+static unsigned grid_y(int n) { return (unsigned)(n < 65535 ? n : 65535); }
+
+hipError_t launch_merge(const MergeArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(merge_kernel, dim3((a.m + 255) / 256, grid_y(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void scale_c_kernel(ScaleCArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.m) return;
+    float2* C = reinterpret_cast<float2*>(a.C);
+    for (int j = blockIdx.y; j < a.n; j += gridDim.y) {
+        const size_t o = (size_t)j * a.ldc + i;
+        if (a.betaZero) {
+            C[o] = make_float2(0.0f, 0.0f);
+        } else {
+            const float2 v = C[o];
+            float2 r;
+            r.x = a.betaR * v.x - a.betaI * v.y;
+            r.y = a.betaR * v.y + a.betaI * v.x;
+            C[o] = r;
+        }
+    }
+}
+
+hipError_t launch_scale_c(const ScaleCArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(scale_c_kernel, dim3((a.m + 255) / 256, grid_y(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ double2 load_op(const float2* X, long long ld, int op, long long row, long long col,
+                                           bool rowsContig)
+{
+    // op(X)(row, col): op N reads X(row, col); op T/C reads X(col, row), C conjugated.
+    const float2 v = (op == 111) ? X[(size_t)col * ld + row] : X[(size_t)row * ld + col];
+    (void)rowsContig;
+    return make_double2((double)v.x, op == 113 ? -(double)v.y : (double)v.y);
+}
+
+__global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const long long waves = ((long long)gridDim.x * 256) >> 6;
+    const float2* A = reinterpret_cast<const float2*>(a.A);
+    const float2* B = reinterpret_cast<const float2*>(a.B);
+    float2* C = reinterpret_cast<float2*>(a.C);
+    for (long long w = wave; w < a.words; w += waves) {
+        unsigned long long bits = a.bitmap[w];
+        while (bits != 0ull) {
+            const int b = __ffsll((long long)bits) - 1;
+            bits &= bits - 1ull;
+            const long long lin = w * 64 + b;
+            const long long i = lin % a.m, j = lin / a.m;
+            double sr = 0.0, si = 0.0;
+            for (long long l = lane; l < a.k; l += 64) {
+                const double2 x = load_op(A, a.lda, a.opA, i, l, false);   // op(A)(i, l)
+                const double2 y = load_op(B, a.ldb, a.opB, l, j, false);   // op(B)(l, j)
+                sr = fma(x.x, y.x, sr);
+                sr = fma(-x.y, y.y, sr);
+                si = fma(x.x, y.y, si);
+                si = fma(x.y, y.x, si);
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                sr += __shfl_xor(sr, d, 64);
+                si += __shfl_xor(si, d, 64);
+            }
+            if (lane == 0) {
+                double cr = a.alphaR * sr - a.alphaI * si;
+                double ci = a.alphaR * si + a.alphaI * sr;
+                if (!a.betaZero) {
+                    const float2 c0 = reinterpret_cast<const float2*>(a.shadow)[lin];
+                    cr += a.betaR * c0.x - a.betaI * c0.y;
+                    ci += a.betaR * c0.y + a.betaI * c0.x;
+                }
+                C[(size_t)j * a.ldc + i] = make_float2((float)cr, (float)ci);
+                if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+            }
+        }
+    }
+}
+
+hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)
+{
+    long long blocks = (a.words + 3) / 4;  // 4 waves per block, one bitmap word per wave
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// FP64 reference: 64 x 64 outputs per 256-thread block, 4 x 4 per thread, k in chunks of
+// 16 staged through LDS. Per block it writes [max rel, sum rel, count, max |c-g|, max |g|].
+__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a)
+{
+    __shared__ double2 As[16][64 + 1];
+    __shared__ double2 Bs[16][64 + 1];
+    __shared__ double red[5][256];
+    const float2* A = reinterpret_cast<const float2*>(a.A);
+    const float2* B = reinterpret_cast<const float2*>(a.B);
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const long long i0 = (long long)blockIdx.x * 64, j0 = (long long)blockIdx.y * 64;
+    double2 acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = make_double2(0.0, 0.0);
+    for (int l0 = 0; l0 < a.k; l0 += 16) {
+        for (int e = tid; e < 16 * 64; e += 256) {
+            int ll, rr;
+            // walk the stored matrix along its contiguous dimension
+            if (a.opA == 111) { rr = e & 63; ll = e >> 6; } else { ll = e & 15; rr = e >> 4; }
+            const long long gi = i0 + rr, gl = l0 + ll;
+            double2 v = make_double2(0.0, 0.0);
+            if (gi < a.m && gl < a.k) v = load_op(A, a.lda, a.opA, gi, gl, false);
+            As[ll][rr] = v;
+            int cc;
+            if (a.opB == 111) { ll = e & 15; cc = e >> 4; } else { cc = e & 63; ll = e >> 6; }
+            const long long gj = j0 + cc, gl2 = l0 + ll;
+            double2 w = make_double2(0.0, 0.0);
+            if (gj < a.n && gl2 < a.k) w = load_op(B, a.ldb, a.opB, gl2, gj, false);
+            Bs[ll][cc] = w;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ll = 0; ll < 16; ++ll) {
+            double2 xa[4], yb[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) xa[x] = As[ll][tx + 16 * x];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) yb[y] = Bs[ll][ty + 16 * y];
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) {
+                    acc[x][y].x = fma(xa[x].x, yb[y].x, acc[x][y].x);
+                    acc[x][y].x = fma(-xa[x].y, yb[y].y, acc[x][y].x);
+                    acc[x][y].y = fma(xa[x].x, yb[y].y, acc[x][y].y);
+                    acc[x][y].y = fma(xa[x].y, yb[y].x, acc[x][y].y);
+                }
+        }
+        __syncthreads();
+    }
+    double mxRel = 0.0, sumRel = 0.0, cnt = 0.0, mxAbs = 0.0, mxRef = 0.0;
+    const float2* C = reinterpret_cast<const float2*>(a.C);
+    const float2* C0 = reinterpret_cast<const float2*>(a.Cinit);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const long long i = i0 + tx + 16 * x, j = j0 + ty + 16 * y;
+            if (i < a.m && j < a.n) {
+                double gr = a.alphaR * acc[x][y].x - a.alphaI * acc[x][y].y;
+                double gi = a.alphaR * acc[x][y].y + a.alphaI * acc[x][y].x;
+                if (!a.betaZero) {
+                    const float2 c0 = C0[(size_t)j * a.ldc + i];
+                    gr += a.betaR * c0.x - a.betaI * c0.y;
+                    gi += a.betaR * c0.y + a.betaI * c0.x;
+                }
+                const float2 c = C[(size_t)j * a.ldc + i];
+                const double dr = (double)c.x - gr, di = (double)c.y - gi;
+                const double err = sqrt(dr * dr + di * di);
+                const double ref = sqrt(gr * gr + gi * gi);
+                if (ref > 0.0) {
+                    const double rel = err / ref;
+                    mxRel = fmax(mxRel, rel);
+                    sumRel += rel;
+                    cnt += 1.0;
+                }
+                mxAbs = fmax(mxAbs, err);
+                mxRef = fmax(mxRef, ref);
+            }
+        }
+    red[0][tid] = mxRel;
+    red[1][tid] = sumRel;
+    red[2][tid] = cnt;
+    red[3][tid] = mxAbs;
+    red[4][tid] = mxRef;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            red[0][tid] = fmax(red[0][tid], red[0][tid + w]);
+            red[1][tid] += red[1][tid + w];
+            red[2][tid] += red[2][tid + w];
+            red[3][tid] = fmax(red[3][tid], red[3][tid + w]);
+            red[4][tid] = fmax(red[4][tid], red[4][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid < 5) a.out[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 5 + tid] = red[tid][0];
+}
+
+hipError_t launch_residual(const ResidualArgs& a, int blocksM, int blocksN, hipStream_t s)
+{
+    hipLaunchKernelGGL(residual_kernel, dim3(blocksM, blocksN), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace c3m

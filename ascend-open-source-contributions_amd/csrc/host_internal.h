@@ -1,0 +1,93 @@
+// host_internal.h -- host-side handle state, plan and helpers shared by the C-ABI files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/aclblas_cgemm3m.h"
+#include "kernel_args.h"
+
+struct _aclblas_handle {
+    hipStream_t stream = nullptr;
+    void* defWs = nullptr;      // library-owned default workspace (grow-only)
+    size_t defWsBytes = 0;
+    void* userWs = nullptr;     // borrowed via aclblasSetWorkspace
+    size_t userWsBytes = 0;
+    int device = -1;            // device the default workspace lives on
+    int cus = 0;                // compute units of that device
+    long long lastPlan[8] = {0, 0, 0, 0, 0, 0, -1, 0};
+    unsigned int* lastCounter = nullptr;  // device counter of flagged outputs (last call)
+    int timing = 0;                       // aclblasCgemm3mSetTiming
+    struct Timed {
+        int stage;
+        hipEvent_t start, stop;
+    };
+    std::vector<Timed> timed;
+    long long timedCalls = 0;
+};
+
+namespace c3m {
+
+constexpr size_t kDefaultWorkspace = 32u << 20;  // cann_ops_blas.h:34-36
+constexpr size_t kAlign = 256;
+constexpr int kVersion = 10000;
+
+void log_error(const char* op, const char* fmt, ...);
+
+enum Stage { ST_PACK = 0, ST_ENERGY = 1, ST_GEMM = 2, ST_MERGE = 3, ST_RECOMPUTE = 4, ST_SCALE = 5, ST_COUNT = 6 };
+
+// Records a HIP event pair around one kernel stage when the handle's timing is enabled.
+class StageTimer {
+  public:
+    StageTimer(aclblasHandle_t h, int stage);
+    ~StageTimer();
+
+  private:
+    aclblasHandle_t h_;
+    int stage_;
+    hipEvent_t start_ = nullptr;
+};
+
+// A full execution plan for one product (pure function of the arguments and CU count).
+struct Plan {
+    int path = 0;         // 0 nothing, 1 fused tile, 2 K-split, 3 beta scale
+    int precision = 0;    // aclblasCgemm3mPrecision_t
+    int format = FMT_F32;
+    int pieces = 1;
+    int bm = 128, bn = 128;
+    int bk = 16;          // k depth of one GEMM stage
+    int parts = 1;
+    int correction = 0;
+    int identity = ID_FUSED;
+    int mPad = 0, nPad = 0, kPad = 0, kSteps = 0;
+    size_t aPlaneBytes = 0, bPlaneBytes = 0;
+    int planesA = 3, planesB = 3;  // planes * pieces per operand
+    // workspace regions (byte offsets)
+    size_t offA = 0, offB = 0, offPartial = 0, offEPA = 0, offEPB = 0, offEA = 0, offEB = 0;
+    size_t offBitmap = 0, offShadow = 0, offCounter = 0, offScaleA = 0, offScaleB = 0;
+    size_t partStride = 0;       // floats per partition (re plane + im plane)
+    long long bitmapWords = 0;
+    int kTiles = 0;
+    size_t total = 0;
+};
+
+struct Problem {
+    aclblasOperation_t ta, tb;
+    int m, n, k;
+    aclblasComplex alpha, beta;
+    const aclblasComplex* A; int lda;
+    const aclblasComplex* B; int ldb;
+    aclblasComplex* C; int ldc;
+};
+
+aclblasStatus_t validate(aclblasHandle_t h, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n, int k,
+                         const aclblasComplex* alpha, int lda, int ldb, const aclblasComplex* beta, int ldc,
+                         const char* op);
+bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan& plan, const char* op);
+aclblasStatus_t ensure_device(aclblasHandle_t h);
+aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, const char* op);
+aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& plan, char* ws, const char* op);
+
+}  // namespace c3m

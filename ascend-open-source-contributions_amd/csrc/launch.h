@@ -1,0 +1,22 @@
+// launch.h -- host-callable launchers of the gfx950 kernels (defined in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernel_args.h"
+
+namespace c3m {
+
+hipError_t launch_pack(const PackArgs& a, hipStream_t s);
+hipError_t launch_pack_pair(const PackArgs& a, const PackArgs& b, hipStream_t s);
+hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s);
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
+hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, hipStream_t s);
+hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, hipStream_t s);
+hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, hipStream_t s);
+hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
+hipError_t launch_scale_c(const ScaleCArgs& a, hipStream_t s);
+hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s);
+hipError_t launch_residual(const ResidualArgs& a, int blocksM, int blocksN, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, long long words, hipStream_t s);
+
+}  // namespace c3m

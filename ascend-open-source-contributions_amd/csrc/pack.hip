@@ -1,0 +1,243 @@
+// pack.hip -- deinterleave/op kernel (HBM-bound): interleaved complex64 -> packed planes.
+//
+// Reference role: K1 gemm_cgemm_deinterleave_kernel (ops-blas/blas/gemm/arch35/
+// cgemm_kernel.cpp:137-157, plane formulas cgemm_deinterleave_common.h:17-51): read each
+// stored element once, apply op (transpose is a layout choice, conjugation negates the
+// imaginary part) and write the three real planes of the 3M identity.
+//
+// gfx950 design: one 256-thread workgroup per 64 (mn) x 64 (k) tile. The load walks the
+// stored matrix along its contiguous dimension (8-byte loads, 512 contiguous bytes per
+// wave), so both op = N and op = T/C read coalesced; the tile is turned in LDS; the store
+// writes whole 1 KiB MFMA fragments (16 bytes per lane, contiguous across the wave) in the
+// format the GEMM kernel copies verbatim. Rows >= mn and k >= kk are written as zeros, so
+// the GEMM never bounds-checks its inner loop. Optional outputs: per-row partial energies
+// sum |x|^2 over the tile's 64 k (FP64, fixed order) for the rounding-correction screen.
+#include "device_util.h"
+#include "launch.h"
+
+namespace c3m {
+
+// Value of plane q for one complex element (re, im) with the imaginary sign already applied.
+__device__ __forceinline__ float plane_value(int role, int identity, int q, float re, float im)
+{
+    if (identity == ID_STANDARD) {
+        return q == 0 ? re : (q == 1 ? im : re + im);
+    }
+    if (role == 0) return q == 0 ? re + im : (q == 1 ? re : im);  // A: (Ar+Ai, Ar, Ai)
+    return q == 0 ? re : (q == 1 ? im - re : re + im);            // B: (Br, Bi-Br, Br+Bi)
+}
+
+__device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, float2 (*tile)[65])
+{
+    const int tid = threadIdx.x;
+    const int r0 = rt * 64, c0 = ct * 64;
+    const float2* src = reinterpret_cast<const float2*>(a.src);
+    for (int e = tid; e < 4096; e += 256) {
+        const int fast = e & 63, slow = e >> 6;
+        const int rl = a.mnContig ? fast : slow;
+        const int cl = a.mnContig ? slow : fast;
+        const int r = r0 + rl, c = c0 + cl;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (r < a.mn && c < a.kk) {
+            v = a.mnContig ? src[(size_t)c * a.ld + r] : src[(size_t)r * a.ld + c];
+            if (a.conj) v.y = -v.y;
+        }
+        tile[rl][cl] = v;
+    }
+    __syncthreads();
+
+    if (a.energyPartial != nullptr && tid < 64) {
+        double s = 0.0;
+        for (int cl = 0; cl < 64; ++cl) {
+            const float2 v = tile[tid][cl];
+            s += (double)v.x * v.x + (double)v.y * v.y;
+        }
+        a.energyPartial[(size_t)ct * a.mnPad + r0 + tid] = s;
+    }
+
+    const size_t mnFrags = (size_t)(a.mnPad / 32);
+    if (a.format == FMT_F32) {
+        // 8 k fragments x 2 mn fragments x 64 lanes; lane holds 4 consecutive k.
+        for (int o = tid; o < 1024; o += 256) {
+            const int kfl = o >> 7, mfl = (o >> 6) & 1, lane = o & 63;
+            const int kf = (c0 >> 3) + kfl;
+            if (kf * 8 >= a.kPad) continue;
+            const int rl = mfl * 32 + (lane & 31);
+            const int cl = kfl * 8 + 4 * (lane >> 5);
+            const size_t idx = ((size_t)kf * mnFrags + (size_t)(r0 >> 5) + mfl) * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                f32x4 out;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const float2 v = tile[rl][cl + s];
+                    out[s] = plane_value(a.role, a.identity, q, v.x, v.y);
+                }
+                reinterpret_cast<f32x4*>(static_cast<char*>(a.planes) + q * a.planeBytes)[idx] = out;
+            }
+        }
+    } else {
+        // 4 k fragments x 2 mn fragments x 64 lanes; lane holds 8 consecutive k (16 bytes).
+        for (int o = tid; o < 512; o += 256) {
+            const int kfl = o >> 7, mfl = (o >> 6) & 1, lane = o & 63;
+            const int kf = (c0 >> 4) + kfl;
+            if (kf * 16 >= a.kPad) continue;
+            const int rl = mfl * 32 + (lane & 31);
+            const int cl = kfl * 16 + 8 * (lane >> 5);
+            const size_t idx = ((size_t)kf * mnFrags + (size_t)(r0 >> 5) + mfl) * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                float x[8];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    const float2 v = tile[rl][cl + s];
+                    x[s] = plane_value(a.role, a.identity, q, v.x, v.y);
+                }
+                if (a.format == FMT_BF16) {
+                    for (int p = 0; p < a.pieces; ++p) {
+                        uint4 w;
+                        uint16_t h[8];
+#pragma unroll
+                        for (int s = 0; s < 8; ++s) {
+                            h[s] = f32_to_bf16_rne(x[s]);
+                            x[s] = x[s] - bf16_to_f32(h[s]);  // exact: remainder of the rounding
+                        }
+                        w.x = h[0] | ((uint32_t)h[1] << 16);
+                        w.y = h[2] | ((uint32_t)h[3] << 16);
+                        w.z = h[4] | ((uint32_t)h[5] << 16);
+                        w.w = h[6] | ((uint32_t)h[7] << 16);
+                        reinterpret_cast<uint4*>(static_cast<char*>(a.planes) +
+                                                 (size_t)(q * a.pieces + p) * a.planeBytes)[idx] = w;
+                    }
+                } else {
+                    const int kb = (c0 + cl) >> 7;
+                    const int e = a.scaleExp[((size_t)q * (a.mnPad >> 7) + (r0 >> 7)) * ((a.kPad + 127) >> 7) + kb];
+                    f16x8 hv;
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) hv[s] = (_Float16)ldexpf(x[s], -e);
+                    reinterpret_cast<f16x8*>(static_cast<char*>(a.planes) + (size_t)q * a.planeBytes)[idx] = hv;
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(PackArgs a, PackArgs b, int blocksA)
+{
+    __shared__ float2 tile[64][65];
+    const bool isA = (int)blockIdx.x < blocksA;
+    const PackArgs& p = isA ? a : b;
+    const int local = isA ? blockIdx.x : blockIdx.x - blocksA;
+    const int rowTiles = p.mnPad / 64;
+    pack_tile(p, local % rowTiles, local / rowTiles, tile);
+}
+
+static int pack_blocks(const PackArgs& a) { return (a.mnPad / 64) * ((a.kPad + 63) / 64); }
+
+hipError_t launch_pack_pair(const PackArgs& a, const PackArgs& b, hipStream_t s)
+{
+    const int na = pack_blocks(a), nb = pack_blocks(b);
+    hipLaunchKernelGGL(pack_kernel, dim3(na + nb), dim3(256), 0, s, a, b, na);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const PackArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(pack_kernel, dim3(pack_blocks(a)), dim3(256), 0, s, a, a, pack_blocks(a));
+    return hipGetLastError();
+}
+
+// FP16 block scale: exponent e per (plane, 128 mn, 128 k) block with max|x| * 2^-e in
+// [2^14, 2^15), so the block's largest value keeps full FP16 precision without overflow.
+__global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)
+{
+    __shared__ float red[3][256];
+    const int mb = blockIdx.x, kb = blockIdx.y;
+    const float2* src = reinterpret_cast<const float2*>(a.src);
+    float mx[3] = {0.0f, 0.0f, 0.0f};
+    for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+        const int fast = e & 127, slow = e >> 7;
+        const int r = mb * 128 + (a.mnContig ? fast : slow);
+        const int c = kb * 128 + (a.mnContig ? slow : fast);
+        if (r < a.mn && c < a.kk) {
+            float2 v = a.mnContig ? src[(size_t)c * a.ld + r] : src[(size_t)r * a.ld + c];
+            if (a.conj) v.y = -v.y;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) mx[q] = fmaxf(mx[q], fabsf(plane_value(a.role, a.identity, q, v.x, v.y)));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = mx[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = fmaxf(red[q][threadIdx.x], red[q][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        const float m = red[threadIdx.x][0];
+        int e = 0;
+        if (m > 0.0f && isfinite(m)) {
+            int ex;
+            frexpf(m, &ex);  // m = f * 2^ex, f in [0.5, 1)
+            e = ex - 15;
+        }
+        const int kBlocks = (a.kPad + 127) >> 7;
+        a.scaleExp[((size_t)threadIdx.x * (a.mnPad >> 7) + mb) * kBlocks + kb] = e;
+    }
+}
+
+hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(scale_exp_kernel, dim3(a.mnPad / 128, (a.kPad + 127) / 128), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Row energies e[r] = sum over k tiles (index order) of the pack kernel's partials; also
+// clears the correction bitmap and the flagged-output counter for this call.
+__global__ __launch_bounds__(256) void energy_kernel(EnergyArgs a)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long r = t; r < a.mnPadA; r += stride) {
+        double s = 0.0;
+        for (int kt = 0; kt < a.kTiles; ++kt) s += a.partialA[(size_t)kt * a.mnPadA + r];
+        a.eA[r] = s;
+    }
+    for (long long r = t; r < a.mnPadB; r += stride) {
+        double s = 0.0;
+        for (int kt = 0; kt < a.kTiles; ++kt) s += a.partialB[(size_t)kt * a.mnPadB + r];
+        a.eB[r] = s;
+    }
+    for (long long w = t; w < a.bitmapWords; w += stride) a.bitmap[w] = 0ull;
+    if (t == 0 && a.counter != nullptr) *a.counter = 0u;
+}
+
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t s)
+{
+    long long work = a.mnPadA > a.mnPadB ? a.mnPadA : a.mnPadB;
+    if (a.bitmapWords > work) work = a.bitmapWords;
+    long long blocks = (work + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(energy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void fill_u64_kernel(unsigned long long* p, long long words)
+{
+    for (long long w = (long long)blockIdx.x * 256 + threadIdx.x; w < words; w += (long long)gridDim.x * 256)
+        p[w] = 0ull;
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, long long words, hipStream_t s)
+{
+    long long blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, words);
+    return hipGetLastError();
+}
+
+}  // namespace c3m

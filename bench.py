@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Cgemm3m throughput on one MI355X: complex TFLOP/s (8mnk) at m = n = k = 8192.
+
+One step = one aclblasCgemm3m call (pack -> three real products -> alpha/beta epilogue ->
+rounding correction) over 8192^3 complex64 operands already resident in HBM, with
+alpha = (1, 0.5), beta = (0.5, -0.25), op = (N, N). `value` is the plain FP32-MFMA path;
+the split-precision forms (bf16x3, bf16x6, fp16 block-scaled) and the (C, T) ops are
+reported beside it, each with its max/mean relative error against an FP64 GEMM of the same
+inputs computed on the GPU, and whether two runs are byte-identical.
+
+Prints progress on stderr and ONE JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "ascend-open-source-contributions_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "complex TFLOP/s (8mnk) Cgemm3m 8192^3: bf16x3 vs FP32-MFMA, rel err vs FP64"
+PEAK = {"fp32": 157.3, "bf16x3": 2516.6, "bf16x6": 2516.6, "fp16": 2516.6}   # dense TFLOP/s
+MFMA_FLOPS_PER_MNK = {"fp32": 6, "bf16x3": 18, "bf16x6": 36, "fp16": 6}       # algorithmic
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=8192)
+    ap.add_argument("--variants", default="fp32,bf16x3,bf16x6,fp16")
+    ap.add_argument("--ops", default="NN,CT")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the operator shards by output blocks with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import cgemm3m
+
+    m = n = k = args.size
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20261203)
+
+    def uniform_c64(count):
+        return (torch.rand(2 * count, generator=gen, device=dev) * 2 - 1).view(torch.complex64)
+
+    A = uniform_c64(m * k)
+    B = uniform_c64(k * n)
+    C0 = uniform_c64(m * n)
+    alpha, beta = complex(1.0, 0.5), complex(0.5, -0.25)
+    stream = torch.cuda.current_stream()
+    h = cgemm3m.Handle(stream=stream)
+    P = cgemm3m.torch_ptr
+
+    def call(cfg, ops, Cbuf, Aop=None):
+        ta, tb = ops[0], ops[1]
+        Ause = A if Aop is None else Aop
+        # square operands: every leading dimension is m
+        return cgemm3m.cgemm3m(h, ta, tb, m, n, k, alpha, P(Ause), m, P(B), m, beta, P(Cbuf), m, cfg, check=False)
+
+    results = {}
+    flops = 8.0 * m * n * k
+    for ops in args.ops.split(","):
+        for name in args.variants.split(","):
+            cfg = cgemm3m.config(name)
+            C = C0.clone()
+            st = call(cfg, ops, C)
+            if st != 0:
+                log("variant %s %s: status %s" % (name, ops, cgemm3m._lib.STATUS.get(st)))
+                results["%s_%s" % (name, ops)] = {"status": cgemm3m._lib.STATUS.get(st, st)}
+                continue
+            for _ in range(args.warmup):
+                call(cfg, ops, C)
+            torch.cuda.synchronize()
+            h.set_timing(True)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            ev0.record(stream)
+            for _ in range(args.steps):
+                call(cfg, ops, C)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) / args.steps * 1e3
+            ev_ms = ev0.elapsed_time(ev1) / args.steps
+            stages = h.timing()
+            h.set_timing(False)
+            calls = max(1, stages.pop("calls"))
+            per_stage = {s: v / calls for s, v in stages.items() if v > 0}
+            r = {"ms_per_step": wall, "ms_events": ev_ms, "tflops": flops / (wall * 1e-3) / 1e12,
+                 "stage_ms": per_stage}
+            gemm_ms = per_stage.get("gemm", 0.0)
+            if gemm_ms > 0:
+                ach = MFMA_FLOPS_PER_MNK[name] * m * n * k / (gemm_ms * 1e-3) / 1e12
+                r["gemm_tflops"] = ach
+                r["gemm_frac_of_peak"] = ach / PEAK[name]
+            if not args.no_errors:
+                Cr = C0.clone()
+                call(cfg, ops, Cr)
+                plan = h.last_plan(sync=True)
+                res = cgemm3m.residual(h, ops[0], ops[1], m, n, k, alpha, P(A), m, P(B), m, beta, P(C0), P(Cr), m)
+                Cr2 = C0.clone()
+                call(cfg, ops, Cr2)
+                torch.cuda.synchronize()
+                r["identical_reruns"] = bool(torch.equal(Cr.view(torch.int32), Cr2.view(torch.int32)))
+                r["err_vs_fp64"] = res
+                r["plan"] = plan
+                del Cr, Cr2
+            results["%s_%s" % (name, ops)] = r
+            log("%s %s: %.3f ms/step, %.1f complex TFLOP/s, stages %s, err %s" % (
+                name, ops, wall, r["tflops"], {s: round(v, 3) for s, v in per_stage.items()},
+                r.get("err_vs_fp64")))
+            del C
+            torch.cuda.empty_cache()
+
+    # Row-scaled stress variant (rows of op(A) scaled by 10^u, u ~ U[-3, 3]): error only.
+    if not args.no_errors and "NN" in args.ops.split(","):
+        u = torch.rand(m, generator=gen, device=dev) * 6 - 3
+        scale = torch.pow(10.0, u).to(torch.complex64)
+        As = (A.view(k, m) * scale.view(1, m)).reshape(-1).contiguous()
+        for name in args.variants.split(","):
+            cfg = cgemm3m.config(name)
+            Cr = C0.clone()
+            if call(cfg, "NN", Cr, Aop=As) != 0:
+                continue
+            res = cgemm3m.residual(h, "N", "N", m, n, k, alpha, P(As), m, P(B), m, beta, P(C0), P(Cr), m)
+            results.setdefault("row_scaled_err_vs_fp64", {})[name] = res
+            log("row-scaled %s err %s" % (name, res))
+        del As
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(A, B, C0, m, n, k, alpha, beta, args.cpu_seconds)
+        log("cpu baseline", cpu)
+
+    main_r = results.get("fp32_NN", {})
+    value = main_r.get("tflops")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_gemm3m_f32_8192.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roof = None
+    if "gemm_tflops" in main_r:
+        roof = {"bound": "mfma", "achieved": round(main_r["gemm_tflops"], 2), "peak": PEAK["fp32"],
+                "unit": "TFLOP/s", "frac": round(main_r["gemm_frac_of_peak"], 4), "traffic": traffic,
+                "kernel": "gemm3m_f32_kernel<128,128> (6mnk FP32 MFMA flops per launch)"}
+    bf = results.get("bf16x3_NN", {})
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3) if value else None,
+        "unit": "complex TFLOP/s (8mnk)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_r.get("ms_per_step", 0.0), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: re/im i.i.d. uniform[-1,1] (seeded torch generator on device)",
+        "config": {"workload": "aclblasCgemm3m m=n=k=%d op=(N,N) alpha=(1,0.5) beta=(0.5,-0.25) "
+                               "column-major complex64, lda=ldb=ldc=%d" % (m, m),
+                   "m": m, "n": n, "k": k},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "bf16x3_over_fp32": (round(bf["tflops"] / value, 3) if bf.get("tflops") and value else None),
+        "variants": results,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(A, B, C0, m, n, k, alpha, beta, seconds):
+    """Netlib-order oracle on the host cores, on the first `cols` output columns of the same
+    inputs (columns are independent in Netlib's order, so this is a bounded sample)."""
+    import numpy as np
+
+    import oracle
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    Ah = A.cpu().numpy()
+    cols = 2
+    rate = None
+    elapsed = 0.0
+    while True:
+        Bh = B[: cols * k].cpu().numpy()
+        Ch = C0[: cols * m].cpu().numpy()
+        t0 = time.perf_counter()
+        oracle.cgemm("N", "N", m, cols, k, alpha, Ah, m, Bh, k, beta, Ch, m, threads=threads)
+        elapsed = time.perf_counter() - t0
+        rate = 8.0 * m * cols * k / elapsed
+        if elapsed * 4 > seconds or cols >= n:
+            break
+        cols = min(n, max(cols * 2, int(cols * seconds / max(elapsed, 1e-3) * 0.9)))
+    return {"value": round(rate / 1e12, 6), "unit": "complex TFLOP/s (8mnk)", "cores": threads, "kind": "port",
+            "sample": "Netlib-order cgemm restatement (oracle/oracle.c, -O2 -ffp-contract=off, OpenMP over "
+                      "columns) on the first %d of %d output columns of the same %d^3 NN inputs, %.1f s"
+                      % (cols, n, m, elapsed)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/*
+ * aclblas_cgemm3m.h -- C ABI of the MI355X (gfx950) complex 3M GEMM library.
+ *
+ * The drop-in surface is the reference's own:
+ *   - types, status codes and operation enums   ops-blas/include/cann_ops_blas.h:22-25,
+ *                                               ops-blas/docs/zh/api_list.md:31-42, :77-79
+ *   - handle functions                          ops-blas/include/cann_ops_blas.h:32-110
+ *   - aclblasCgemm3m                            ops-blas/include/cann_ops_blas.h:497-500,
+ *                                               ops-blas/blas/gemm/arch35/cgemm3m_host.cpp:17-23
+ * Everything after the "extensions" banner has no counterpart in the reference; each
+ * declaration says which caller it is for.
+ *
+ * Conventions (same as the reference): matrices are column-major complex64 in device
+ * memory owned by the caller; alpha/beta are HOST pointers read during the call; work
+ * is enqueued on the handle's stream and the call returns before it completes.
+ * One handle must not be used by two host threads at once.
+ */
+#ifndef ACLBLAS_CGEMM3M_H_
+#define ACLBLAS_CGEMM3M_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ops-blas/docs/zh/api_list.md:31-42 (values are part of the ABI). */
+typedef enum {
+    ACLBLAS_STATUS_SUCCESS = 0,
+    ACLBLAS_STATUS_NOT_INITIALIZED = 1,
+    ACLBLAS_STATUS_ALLOC_FAILED = 2,
+    ACLBLAS_STATUS_INVALID_VALUE = 3,
+    ACLBLAS_STATUS_MAPPING_ERROR = 4,
+    ACLBLAS_STATUS_EXECUTION_FAILED = 5,
+    ACLBLAS_STATUS_INTERNAL_ERROR = 6,
+    ACLBLAS_STATUS_NOT_SUPPORTED = 7,
+    ACLBLAS_STATUS_ARCH_MISMATCH = 8,
+    ACLBLAS_STATUS_HANDLE_IS_NULLPTR = 9,
+    ACLBLAS_STATUS_INVALID_ENUM = 10,
+    ACLBLAS_STATUS_UNKNOWN = 11
+} aclblasStatus_t;
+
+/* ops-blas/docs/zh/api_list.md:77-79. */
+typedef enum {
+    ACLBLAS_OP_N = 111,
+    ACLBLAS_OP_T = 112,
+    ACLBLAS_OP_C = 113
+} aclblasOperation_t;
+
+/* Interleaved complex64: 8 bytes, 4-byte aligned (fields used at
+ * ops-blas/blas/gemm/arch35/cgemm_host.cpp:719-720). */
+typedef struct {
+    float real;
+    float imag;
+} aclblasComplex;
+
+/* ops-blas/include/cann_ops_blas.h:22-25: opaque handle. */
+typedef struct _aclblas_handle* aclblasHandle_t;
+typedef struct _aclblas_handle* aclblasHandle;
+
+/* The reference's aclrtStream; here it carries a hipStream_t (nullptr = default stream). */
+typedef void* aclrtStream;
+
+/* ops-blas/include/cann_ops_blas.h:23: log sink for error text. */
+typedef void (*aclblasLogCallback)(const char*);
+
+/* ops-blas/include/cann_ops_blas.h:32-45. *handle must be nullptr on entry.
+ * The 32 MiB default workspace is reserved on first use rather than here, so a handle
+ * can be created (and argument validation exercised) on a host without a GPU. */
+aclblasStatus_t aclblasCreate(aclblasHandle_t* handle);
+
+/* ops-blas/include/cann_ops_blas.h:47-58. Synchronises the stream, frees the default
+ * workspace; a user workspace is forgotten, not freed. */
+aclblasStatus_t aclblasDestroy(aclblasHandle_t handle);
+
+/* ops-blas/include/cann_ops_blas.h:60-71. Switching streams reverts to the default workspace. */
+aclblasStatus_t aclblasSetStream(aclblasHandle_t handle, aclrtStream stream);
+
+/* ops-blas/include/cann_ops_blas.h:73-82. */
+aclblasStatus_t aclblasGetStream(aclblasHandle_t handle, aclrtStream* stream);
+
+/* ops-blas/include/cann_ops_blas.h:84-98. Borrowed, grow-only. When a call needs more
+ * than the user workspace holds it fails with ACLBLAS_STATUS_EXECUTION_FAILED
+ * (ops-blas/docs/zh/api_list.md:203); query the need with aclblasCgemm3mGetWorkspaceSize. */
+aclblasStatus_t aclblasSetWorkspace(aclblasHandle_t handle, void* workspace, size_t workspaceSize);
+
+/* ops-blas/include/cann_ops_blas.h:100-110. MAJOR*10000 + MINOR*100 + PATCH. */
+aclblasStatus_t aclblasGetVersion(aclblasHandle_t handle, int* version);
+
+/* C <- alpha*op(A)*op(B) + beta*C with three real products.
+ * ops-blas/include/cann_ops_blas.h:497-500; contract ops-blas/blas/gemm/README.md:909-941;
+ * validation order ops-blas/blas/gemm/arch35/cgemm_host.cpp:589-634, :687-730. */
+aclblasStatus_t aclblasCgemm3m(aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb,
+                               int m, int n, int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                               const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
+                               int ldc);
+
+/* ========================= extensions (no reference counterpart) ========================= */
+
+/* Arithmetic used for the three real products. FP32 is what aclblasCgemm3m runs. */
+typedef enum {
+    ACLBLAS_CGEMM3M_FP32 = 0,        /* FP32 MFMA, FP32 accumulation (+ FP64 correction pass) */
+    ACLBLAS_CGEMM3M_BF16X3 = 1,      /* each FP32 plane split hi+lo BF16; hi*hi + hi*lo + lo*hi */
+    ACLBLAS_CGEMM3M_FP16_SCALED = 2, /* FP16 planes with a power-of-two scale per 128x128 block */
+    ACLBLAS_CGEMM3M_BF16X6 = 3       /* three-piece BF16 split, six products (more accurate) */
+} aclblasCgemm3mPrecision_t;
+
+/* Planner overrides; 0 everywhere means "let the planner/autotuned table decide". */
+typedef struct {
+    int precision;  /* aclblasCgemm3mPrecision_t */
+    int tile;       /* 0 auto, 64 or 128: output tile edge */
+    int kSplit;     /* 0 auto, else 1..16 K partitions reduced in fixed order */
+    int correction; /* 0 auto, 1 force on, -1 force off (FP32 only) */
+    int identity;   /* 0 auto (fused identity), 1 fused, 2 standard 3M identity */
+} aclblasCgemm3mConfig;
+
+/* For callers that choose the arithmetic or pin the plan (bench, tests, autotuner, the
+ * torch autograd wrapper). Same argument contract as aclblasCgemm3m; config may be null. */
+aclblasStatus_t aclblasCgemm3mEx(aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb,
+                                 int m, int n, int k, const aclblasComplex* alpha, const aclblasComplex* A,
+                                 int lda, const aclblasComplex* B, int ldb, const aclblasComplex* beta,
+                                 aclblasComplex* C, int ldc, const aclblasCgemm3mConfig* config);
+
+/* For callers that must size a user workspace up front (hipGraph capture, serving loops):
+ * bytes the call with these arguments would need. */
+aclblasStatus_t aclblasCgemm3mGetWorkspaceSize(aclblasHandle_t handle, aclblasOperation_t transa,
+                                               aclblasOperation_t transb, int m, int n, int k,
+                                               const aclblasComplex* alpha, const aclblasComplex* beta,
+                                               const aclblasCgemm3mConfig* config, size_t* bytes);
+
+/* For callers with many small independent problems: `count` problems in one persistent
+ * launch (plus one packing launch and, if any problem is K-split, one merge launch).
+ * All arrays are HOST arrays of length count; A/B/C entries are device pointers.
+ * Per-problem validation follows aclblasCgemm3m; the first invalid problem's status is
+ * returned and nothing is launched. precision is an aclblasCgemm3mPrecision_t. */
+aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
+                                             const aclblasOperation_t* transb, const int* m, const int* n,
+                                             const int* k, const aclblasComplex* alpha,
+                                             const aclblasComplex* const* A, const int* lda,
+                                             const aclblasComplex* const* B, const int* ldb,
+                                             const aclblasComplex* beta, aclblasComplex* const* C, const int* ldc,
+                                             int precision);
+
+/* Workspace bytes aclblasCgemm3mGroupedBatched needs for these problems. */
+aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasHandle_t handle, int count,
+                                                             const aclblasOperation_t* transa,
+                                                             const aclblasOperation_t* transb, const int* m,
+                                                             const int* n, const int* k, int precision,
+                                                             size_t* bytes);
+
+/* Error report for benches and tests: FP64 complex GEMM of the same inputs on the GPU,
+ * compared with C (the FP32 result already computed from C0 = Cinit).
+ * out[0] = max |c - g| / |g| over g != 0 (elementwise, complex modulus)
+ * out[1] = mean of the same
+ * out[2] = max |c - g| / max |g|       (normwise)
+ * out[3] = max |g|
+ * Cinit is the C before the GEMM (may be null when beta == 0). Synchronises the stream. */
+aclblasStatus_t aclblasCgemm3mResidual(aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb,
+                                       int m, int n, int k, const aclblasComplex* alpha, const aclblasComplex* A,
+                                       int lda, const aclblasComplex* B, int ldb, const aclblasComplex* beta,
+                                       const aclblasComplex* Cinit, const aclblasComplex* C, int ldc,
+                                       double* out);
+
+/* Diagnostics of the last aclblasCgemm3m/Ex call on this handle (host-side plan; the
+ * flagged count needs a stream sync and is read only when `sync` is nonzero).
+ * info[0] path (0 none, 1 fused-tile, 2 K-split, 3 alpha-zero/beta scale)
+ * info[1] tile edge, info[2] K partitions, info[3] correction enabled,
+ * info[4] precision, info[5] identity (1 fused, 2 standard), info[6] flagged outputs
+ * (-1 if not read), info[7] workspace bytes used. */
+aclblasStatus_t aclblasCgemm3mGetLastPlan(aclblasHandle_t handle, int sync, long long* info);
+
+/* For benches and profilers: while enabled, every later call on this handle records a HIP
+ * event pair around each kernel stage on the handle's stream. */
+aclblasStatus_t aclblasCgemm3mSetTiming(aclblasHandle_t handle, int enable);
+
+/* Milliseconds per stage summed over the calls recorded since the last read, then reset:
+ * ms[0] pack, ms[1] energy, ms[2] products (GEMM), ms[3] merge, ms[4] recompute,
+ * ms[5] beta scale; *calls = number of calls. Synchronises the recorded events. */
+aclblasStatus_t aclblasCgemm3mGetTiming(aclblasHandle_t handle, double* ms, long long* calls);
+
+/* Route error text to a callback (null restores stderr). Process-wide. */
+aclblasStatus_t aclblasSetLogCallback(aclblasLogCallback callback);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACLBLAS_CGEMM3M_H_ */

@@ -1,0 +1,130 @@
+"""CPU tests of the C ABI: the library loads, exports every declared symbol, and the
+argument contract (status codes and their order) holds before any device work.
+
+Mirrors the reference's ED rows (ops-blas/test/gemm/cgemm3m/arch35/cgemm3m_test.csv:307-338)
+and NullHandle (cgemm3m_test.cpp:166-173); validation order is cgemm_host.cpp:589-634, 701-730.
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+import cgemm3m
+from cgemm3m import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FAKE = ctypes.c_void_p(0x1000)  # never dereferenced: every case below fails or returns before use
+
+
+def test_header_symbols_exported():
+    L = _lib.load()
+    header = open(os.path.join(ROOT, "include", "aclblas_cgemm3m.h")).read()
+    declared = set(re.findall(r"\b(aclblas\w+)\s*\(", header))
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_handle_lifecycle():
+    L = _lib.load()
+    assert L.aclblasCreate(None) == _lib.HANDLE_IS_NULLPTR
+    h = ctypes.c_void_p()
+    assert L.aclblasCreate(ctypes.byref(h)) == _lib.SUCCESS and h.value
+    assert L.aclblasCreate(ctypes.byref(h)) == _lib.INVALID_VALUE  # *handle must be null
+    s = ctypes.c_void_p(1)
+    assert L.aclblasGetStream(h, ctypes.byref(s)) == _lib.SUCCESS and not s.value
+    assert L.aclblasGetStream(h, None) == _lib.INVALID_VALUE
+    assert L.aclblasSetStream(None, None) == _lib.HANDLE_IS_NULLPTR
+    assert L.aclblasSetWorkspace(h, None, 16) == _lib.INVALID_VALUE
+    assert L.aclblasSetWorkspace(h, FAKE, 0) == _lib.INVALID_VALUE
+    assert L.aclblasSetWorkspace(None, FAKE, 16) == _lib.HANDLE_IS_NULLPTR
+    v = ctypes.c_int()
+    assert L.aclblasGetVersion(None, ctypes.byref(v)) == _lib.SUCCESS and v.value == 10000
+    assert L.aclblasGetVersion(h, None) == _lib.INVALID_VALUE
+    assert L.aclblasDestroy(h) == _lib.SUCCESS
+    assert L.aclblasDestroy(None) == _lib.HANDLE_IS_NULLPTR
+
+
+@pytest.fixture
+def h():
+    hd = cgemm3m.Handle()
+    logs = []
+    cb = _lib.LOG_CB(lambda msg: logs.append(msg.decode()))
+    hd.L.aclblasSetLogCallback(cb)
+    hd.logs = logs
+    yield hd
+    hd.L.aclblasSetLogCallback(_lib.LOG_CB())
+    hd.close()
+
+
+def call(h, ta="N", tb="N", m=16, n=16, k=16, alpha=1, A=FAKE, lda=16, B=FAKE, ldb=16, beta=0, C=FAKE, ldc=16):
+    return cgemm3m.cgemm3m(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, check=False)
+
+
+def test_null_handle():
+    L = _lib.load()
+    one, zero = _lib.Complex(1, 0), _lib.Complex(0, 0)
+    st = L.aclblasCgemm3m(None, 111, 111, 0, 0, 0, ctypes.byref(one), None, 1, None, 1, ctypes.byref(zero), None, 1)
+    assert st == _lib.HANDLE_IS_NULLPTR
+
+
+@pytest.mark.parametrize("kw,expect", [
+    (dict(m=0, lda=1, ldc=1), 0),                          # m0_quick_return
+    (dict(n=0), 0),                                        # n0_quick_return
+    (dict(m=0, n=0, k=0, lda=1, ldb=1, ldc=1), 0),         # m0n0k0
+    (dict(m=0, A=None, lda=1, ldc=1), 0),                  # m0_a_null
+    (dict(n=0, B=None), 0),                                # n0_b_null
+    (dict(m=0, C=None, lda=1, ldc=1, beta=1), 0),          # m0_c_null
+    (dict(k=0, A=None, B=None, ldb=1, beta=1), 0),         # k0 with beta 1: nothing to do
+    (dict(alpha=0, A=None, beta=1), 0),                    # alpha0_a_null, beta 1
+    (dict(alpha=0, A=None, B=None, beta=1), 0),
+    (dict(alpha=None), 3),                                 # null_alpha
+    (dict(beta=None), 3),                                  # null_beta
+    (dict(A=None), 3),                                     # null_A
+    (dict(B=None), 3),                                     # null_B
+    (dict(C=None, beta=1), 3),                             # null_C
+    (dict(ta=999), 3),                                     # invalid_transa (INVALID_ENUM remapped)
+    (dict(tb=999), 3),                                     # invalid_transb
+    (dict(lda=8), 3),                                      # invalid_lda_N
+    (dict(ta="T", lda=8), 3),                              # invalid_lda_T
+    (dict(ldb=8), 3),                                      # invalid_ldb_N
+    (dict(tb="T", ldb=8), 3),                              # invalid_ldb_T
+    (dict(ldc=8), 3),                                      # invalid_ldc
+    (dict(m=-1, lda=1, ldc=1), 3),                         # neg_m
+    (dict(n=-1), 3),                                       # neg_n
+    (dict(k=-1, ldb=1), 3),                                # neg_k
+    (dict(m=-1, A=None, lda=1, ldc=1), 3),                 # neg_m_null_A
+    (dict(ta="T", m=20, k=4, lda=3, ldc=20), 3),           # T: lda >= k
+    (dict(ta="T", m=20, k=4, lda=4, ldc=20, A=None), 3),   # lda ok, then null A
+    (dict(tb="C", n=20, k=4, ldb=19, ldc=16), 3),          # C: ldb >= n
+    (dict(ta="N", m=0, n=0, lda=0, ldc=1), 3),             # lda >= max(1, m)
+])
+def test_status_contract(h, kw, expect):
+    assert call(h, **kw) == expect
+
+
+def test_validation_order(h):
+    """Enum errors come before dimension errors, which come before pointer errors."""
+    assert call(h, ta=999, m=-1) == 3
+    assert call(h, m=-1, alpha=None) == 3
+    assert call(None if False else h, m=16, lda=1, C=None) == 3
+    assert any("lda=1" in x for x in h.logs)
+
+
+def test_config_validation(h):
+    for cfg in (cgemm3m.config(precision=9), cgemm3m.config(tile=96), cgemm3m.config(k_split=17),
+                cgemm3m.config(correction=2), cgemm3m.config(identity=5)):
+        st = cgemm3m.cgemm3m(h, "N", "N", 16, 16, 16, 1, FAKE, 16, FAKE, 16, 0, FAKE, 16, cfg, check=False)
+        assert st == _lib.INVALID_VALUE
+
+
+def test_workspace_size_query(h):
+    n = cgemm3m.workspace_size(h, "N", "N", 256, 256, 256, 1, 0)
+    # 3 A planes + 3 B planes of 256 x 256 FP32 at least
+    assert n >= 6 * 256 * 256 * 4
+    big = cgemm3m.workspace_size(h, "N", "N", 8192, 8192, 8192, 1, 0.5)
+    assert big >= 6 * 8192 * 8192 * 4
+    bf = cgemm3m.workspace_size(h, "N", "N", 8192, 8192, 8192, 1, 0, cgemm3m.config("bf16x3"))
+    assert bf >= 6 * 8192 * 8192 * 4
+    assert cgemm3m.workspace_size(h, "N", "N", 0, 5, 5, 1, 0) == 0
