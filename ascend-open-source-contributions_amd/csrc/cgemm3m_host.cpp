@@ -141,9 +141,9 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     double unit = 0x1p-24;  // unit roundoff of the product arithmetic (correction screen)
     switch (pl.precision) {
         case ACLBLAS_CGEMM3M_FP32: pl.format = FMT_F32; pl.pieces = 1; pl.bk = 16; break;
-        case ACLBLAS_CGEMM3M_BF16X3: pl.format = FMT_BF16; pl.pieces = 2; pl.bk = 32; unit = 0x1p-16; break;
-        case ACLBLAS_CGEMM3M_BF16X6: pl.format = FMT_BF16; pl.pieces = 3; pl.bk = 32; break;
-        case ACLBLAS_CGEMM3M_FP16_SCALED: pl.format = FMT_F16; pl.pieces = 1; pl.bk = 32; unit = 0x1p-11; break;
+        case ACLBLAS_CGEMM3M_BF16X3: pl.format = FMT_BF16; pl.pieces = 2; pl.bk = 16; unit = 0x1p-16; break;
+        case ACLBLAS_CGEMM3M_BF16X6: pl.format = FMT_BF16; pl.pieces = 3; pl.bk = 16; break;
+        case ACLBLAS_CGEMM3M_FP16_SCALED: pl.format = FMT_F16; pl.pieces = 1; pl.bk = 16; unit = 0x1p-11; break;
         default: log_error(op, "invalid precision %d", pl.precision); return false;
     }
     (void)unit;
@@ -164,6 +164,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     } else {
         tile = tiles_for(128) >= cus ? 128 : 64;
     }
+    // The FP16 block-scaled kernel carries two accumulator sets and only has a 64 tile.
+    if (pl.format == FMT_F16) tile = 64;
     pl.bm = pl.bn = tile;
     pl.mPad = (int)round_up((size_t)m, 128);
     pl.nPad = (int)round_up((size_t)n, 128);

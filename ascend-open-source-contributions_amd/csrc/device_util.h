@@ -88,4 +88,60 @@ __device__ __forceinline__ void flag_output(unsigned long long* bitmap, long lon
     atomicOr(&bitmap[lin >> 6], 1ull << (lin & 63));
 }
 
+// Output stage shared by the product kernels. acc[q][a][b] is the 32x32 FP32 tile of
+// product q for the wave's a-th column tile and b-th row tile; the lane holds output row
+// iBase + 32*b and, in register r, column jBase + 32*a + 8*(r/4) + (r%4).
+// PARTIAL: write this K partition's (re, im) planes (ld = mPad, padded extent, no checks).
+// Otherwise: alpha/beta in registers, C read only when beta != 0, rows >= m and columns
+// >= n never touched, and the rounding-correction screen on every written output.
+template <int TN, int TM, bool PARTIAL>
+__device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[3][TN][TM], int part, int iBase,
+                                           int jBase)
+{
+    if (PARTIAL) {
+        float* pr = g.partial + (size_t)part * g.partStride;
+        float* pi = pr + (size_t)g.mPad * g.nPad;
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = iBase + b * 32;
+                    const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+                    float re, im;
+                    combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[2][a][b][r], re, im);
+                    const size_t o = (size_t)j * g.mPad + i;
+                    pr[o] = re;
+                    pi[o] = im;
+                }
+    } else {
+        float2* C = reinterpret_cast<float2*>(g.C);
+        const bool bz = g.betaZero != 0;
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = iBase + b * 32;
+                    const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+                    if (i < g.m && j < g.n) {
+                        float re, im;
+                        combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[2][a][b][r], re, im);
+                        const size_t o = (size_t)j * g.ldc + i;
+                        float2 c0 = make_float2(0.0f, 0.0f);
+                        if (!bz) c0 = C[o];
+                        const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0);
+                        if (g.correction && needs_recompute(c, g.eA[i], g.eB[j], g.corrScale, g.corrTau)) {
+                            const long long lin = (long long)j * g.m + i;
+                            flag_output(g.bitmap, lin);
+                            if (!bz) reinterpret_cast<float2*>(g.shadow)[lin] = c0;
+                        }
+                        C[o] = c;
+                    }
+                }
+    }
+}
+
 }  // namespace c3m

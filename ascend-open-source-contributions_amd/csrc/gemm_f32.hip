@@ -140,50 +140,7 @@ __global__ __launch_bounds__(256, 1) void gemm3m_f32_kernel(GemmArgs g)
     // j = ... + 8*(r/4) + 4*(lane/32) + r%4 of each 32x32 tile.
     const int iBase = tmI * BM + wm * (BM / 2) + (lane & 31);
     const int jBase = tnI * BN + wn * (BN / 2) + 4 * (lane >> 5);
-    if (PARTIAL) {
-        float* pr = g.partial + (size_t)part * g.partStride;
-        float* pi = pr + (size_t)g.mPad * g.nPad;
-#pragma unroll
-        for (int a = 0; a < T::TN; ++a)
-#pragma unroll
-            for (int b = 0; b < T::TM; ++b)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int i = iBase + b * 32;
-                    const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
-                    float re, im;
-                    combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[2][a][b][r], re, im);
-                    const size_t o = (size_t)j * g.mPad + i;
-                    pr[o] = re;
-                    pi[o] = im;
-                }
-    } else {
-        float2* C = reinterpret_cast<float2*>(g.C);
-        const bool bz = g.betaZero != 0;
-#pragma unroll
-        for (int a = 0; a < T::TN; ++a)
-#pragma unroll
-            for (int b = 0; b < T::TM; ++b)
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int i = iBase + b * 32;
-                    const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
-                    if (i < g.m && j < g.n) {
-                        float re, im;
-                        combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[2][a][b][r], re, im);
-                        const size_t o = (size_t)j * g.ldc + i;
-                        float2 c0 = make_float2(0.0f, 0.0f);
-                        if (!bz) c0 = C[o];
-                        const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0);
-                        if (g.correction && needs_recompute(c, g.eA[i], g.eB[j], g.corrScale, g.corrTau)) {
-                            const long long lin = (long long)j * g.m + i;
-                            flag_output(g.bitmap, lin);
-                            if (!bz) reinterpret_cast<float2*>(g.shadow)[lin] = c0;
-                        }
-                        C[o] = c;
-                    }
-                }
-    }
+    store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
 }
 
 template <int BM, int BN, bool PARTIAL>

@@ -1,9 +1,218 @@
-// gemm_lowp.hip -- split-precision products (BF16 pieces, FP16 block-scaled). Placeholder
-// launchers until the kernels land; the host reports NOT_SUPPORTED through them.
+// gemm_lowp.hip -- the three real products on 16-bit MFMA (v_mfma_f32_32x32x16_{bf16,f16}).
+//
+// Same block/wave decomposition, packed-fragment LDS ring and epilogue as the FP32 kernel
+// (gemm_f32.hip); one stage is one 16-deep k fragment.
+//
+//  BF16 split (NP = 2, "bf16x3"): each FP32 plane value x = hi + lo + r with hi = bf16(x),
+//    lo = bf16(x - hi) (the pack kernel does the split). Per product
+//    P = a_hi*b_hi + a_hi*b_lo + a_lo*b_hi, three MFMAs into one FP32 accumulator; a_lo*b_lo
+//    and the remainders r are dropped (about 2^-16 relative per operand).
+//  BF16 split (NP = 3, "bf16x6"): x = hi + mid + lo; six MFMAs (hh, hm, mh, hl, lh, mm) keep
+//    every term down to 2^-24 relative (mid*lo, lo*mid, lo*lo dropped).
+//  FP16 block-scaled (F16S): each 128 (mn) x 128 (k) block of every plane was stored as
+//    fp16(x * 2^-e) with its own exponent e; the MFMAs of one 128-deep k block accumulate
+//    into a scratch accumulator that is added to the main one times 2^(eA + eB) (exact
+//    power-of-two scaling), so the block scales can differ along k.
 #include "device_util.h"
 #include "launch.h"
 
 namespace c3m {
-hipError_t launch_gemm_bf16(const GemmArgs&, int, int, int, hipStream_t) { return hipErrorNotSupported; }
-hipError_t launch_gemm_f16(const GemmArgs&, int, int, hipStream_t) { return hipErrorNotSupported; }
+
+template <int BM, int BN, int NP>
+struct LowTile {
+    static constexpr int TM = BM / 64, TN = BN / 64;
+    static constexpr int FA = BM / 32, FB = BN / 32;
+    static constexpr int A16 = 3 * NP * FA * 64;  // 16-byte lane vectors per stage, A side
+    static constexpr int B16 = 3 * NP * FB * 64;
+    static constexpr int S16 = A16 + B16;
+    static constexpr int LOADS = (S16 + 255) / 256;
+};
+
+__device__ __forceinline__ f32x16 mfma16(uint4 a, uint4 b, f32x16 c, bool half)
+{
+    if (half)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
+                                                      0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+template <int BM, int BN, int NP, bool F16S, bool PARTIAL>
+__global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
+{
+    using T = LowTile<BM, BN, NP>;
+    __shared__ uint4 lds[2 * T::S16];
+
+    int part, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int ks0 = (int)((long long)g.kSteps * part / g.parts);
+    const int ks1 = (int)((long long)g.kSteps * (part + 1) / g.parts);
+
+    const uint4* Ap = reinterpret_cast<const uint4*>(g.Ap);
+    const uint4* Bp = reinterpret_cast<const uint4*>(g.Bp);
+    const size_t aPlane = g.aPlaneBytes / 16, bPlane = g.bPlaneBytes / 16;
+    const size_t aRow = (size_t)(g.mPad / 32) * 64;  // lane vectors per 16-deep k fragment
+    const size_t bRow = (size_t)(g.nPad / 32) * 64;
+
+    const uint4* src[T::LOADS];
+    size_t step[T::LOADS];
+    bool live[T::LOADS];
+#pragma unroll
+    for (int u = 0; u < T::LOADS; ++u) {
+        const int e = tid + 256 * u;
+        live[u] = e < T::S16;
+        if (e < T::A16) {
+            const int row = e / (T::FA * 64);  // plane q * NP + piece
+            const int w = e % (T::FA * 64);
+            src[u] = Ap + row * aPlane + (size_t)tmI * T::FA * 64 + w;
+            step[u] = aRow;
+        } else {
+            const int eb = live[u] ? e - T::A16 : 0;
+            const int row = eb / (T::FB * 64);
+            const int w = eb % (T::FB * 64);
+            src[u] = Bp + row * bPlane + (size_t)tnI * T::FB * 64 + w;
+            step[u] = bRow;
+        }
+    }
+
+    f32x16 acc[3][T::TN][T::TM];
+    f32x16 blk[F16S ? 3 : 1][F16S ? T::TN : 1][F16S ? T::TM : 1];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+            for (int b = 0; b < T::TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    acc[q][a][b][r] = 0.0f;
+                    if (F16S) blk[q][a][b][r] = 0.0f;
+                }
+
+    uint4 gbuf[T::LOADS];
+    if (ks0 < ks1) {
+#pragma unroll
+        for (int u = 0; u < T::LOADS; ++u)
+            if (live[u]) gbuf[u] = src[u][(size_t)ks0 * step[u]];
+#pragma unroll
+        for (int u = 0; u < T::LOADS; ++u)
+            if (live[u]) lds[tid + 256 * u] = gbuf[u];
+    }
+    __syncthreads();
+
+    for (int ks = ks0; ks < ks1; ++ks) {
+        const int buf = (ks - ks0) & 1;
+        const bool more = ks + 1 < ks1;
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u)
+                if (live[u]) gbuf[u] = src[u][(size_t)(ks + 1) * step[u]];
+        }
+        const uint4* L = lds + buf * T::S16;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            uint4 af[NP][T::TM], bf[NP][T::TN];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                for (int b = 0; b < T::TM; ++b) af[p][b] = L[((q * NP + p) * T::FA + wm * T::TM + b) * 64 + lane];
+#pragma unroll
+                for (int a = 0; a < T::TN; ++a)
+                    bf[p][a] = L[T::A16 + ((q * NP + p) * T::FB + wn * T::TN + a) * 64 + lane];
+            }
+#pragma unroll
+            for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                for (int b = 0; b < T::TM; ++b) {
+                    if (F16S) {
+                        blk[q][a][b] = mfma16(bf[0][a], af[0][b], blk[q][a][b], true);
+                    } else if (NP == 2) {
+                        // small terms first, then hi*hi
+                        acc[q][a][b] = mfma16(bf[1][a], af[0][b], acc[q][a][b], false);
+                        acc[q][a][b] = mfma16(bf[0][a], af[1][b], acc[q][a][b], false);
+                        acc[q][a][b] = mfma16(bf[0][a], af[0][b], acc[q][a][b], false);
+                    } else {
+                        // NP == 3: pieces 0 = hi, 1 = mid, 2 = lo; the six products down to 2^-24
+                        acc[q][a][b] = mfma16(bf[1][a], af[1][b], acc[q][a][b], false);  // mid*mid
+                        acc[q][a][b] = mfma16(bf[2][a], af[0][b], acc[q][a][b], false);  // lo*hi
+                        acc[q][a][b] = mfma16(bf[0][a], af[2][b], acc[q][a][b], false);  // hi*lo
+                        acc[q][a][b] = mfma16(bf[1][a], af[0][b], acc[q][a][b], false);  // mid*hi
+                        acc[q][a][b] = mfma16(bf[0][a], af[1][b], acc[q][a][b], false);  // hi*mid
+                        acc[q][a][b] = mfma16(bf[0][a], af[0][b], acc[q][a][b], false);  // hi*hi
+                    }
+                }
+        }
+        if (F16S) {
+            // End of a 128-deep k block (or of this partition): fold the block's products in.
+            const int kb = (ks * 16) >> 7;
+            if (!more || (((ks + 1) * 16) >> 7) != kb) {
+                const int mb = (tmI * BM) >> 7, nb = (tnI * BN) >> 7;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const int e = g.scaleA[((size_t)q * (g.mPad >> 7) + mb) * g.kBlocks128 + kb] +
+                                  g.scaleB[((size_t)q * (g.nPad >> 7) + nb) * g.kBlocks128 + kb];
+#pragma unroll
+                    for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                        for (int b = 0; b < T::TM; ++b)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r) {
+                                acc[q][a][b][r] += ldexpf(blk[q][a][b][r], e);
+                                blk[q][a][b][r] = 0.0f;
+                            }
+                }
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u)
+                if (live[u]) lds[(buf ^ 1) * T::S16 + tid + 256 * u] = gbuf[u];
+        }
+        __syncthreads();
+    }
+
+    const int iBase = tmI * BM + wm * (BM / 2) + (lane & 31);
+    const int jBase = tnI * BN + wn * (BN / 2) + 4 * (lane >> 5);
+    store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
+}
+
+template <int BM, int BN, int NP, bool F16S, bool PARTIAL>
+static hipError_t launch_one(const GemmArgs& g, hipStream_t s)
+{
+    const int blocks = g.tilesM * g.tilesN * g.parts;
+    hipLaunchKernelGGL((gemm3m_lowp_kernel<BM, BN, NP, F16S, PARTIAL>), dim3(blocks), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+template <int NP>
+static hipError_t launch_bf16_np(const GemmArgs& g, int bm, int bn, hipStream_t s)
+{
+    const bool partial = g.parts > 1;
+    if (bm == 128 && bn == 128)
+        return partial ? launch_one<128, 128, NP, false, true>(g, s) : launch_one<128, 128, NP, false, false>(g, s);
+    if (bm == 64 && bn == 64)
+        return partial ? launch_one<64, 64, NP, false, true>(g, s) : launch_one<64, 64, NP, false, false>(g, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, hipStream_t s)
+{
+    if (pieces == 2) return launch_bf16_np<2>(g, bm, bn, s);
+    if (pieces == 3) return launch_bf16_np<3>(g, bm, bn, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, hipStream_t s)
+{
+    const bool partial = g.parts > 1;
+    // Two accumulator sets (running + current k block): the 128x128 tile would not fit.
+    (void)bm;
+    (void)bn;
+    return partial ? launch_one<64, 64, 1, true, true>(g, s) : launch_one<64, 64, 1, true, false>(g, s);
+}
+
 }  // namespace c3m
