@@ -127,3 +127,47 @@ def workspace_size(handle, transa, transb, m, n, k, alpha, beta, cfg=None):
     if st != _lib.SUCCESS:
         raise Cgemm3mError(st, "aclblasCgemm3mGetWorkspaceSize")
     return n_bytes.value
+
+
+def _arrays(probs):
+    cnt = len(probs)
+    I = ctypes.c_int * cnt
+    CX = _lib.Complex * cnt
+    VP = ctypes.c_void_p * cnt
+
+    def ints(key, f=int):
+        return I(*[f(p[key]) for p in probs])
+
+    def cx(key):
+        return CX(*[_lib.Complex(complex(p[key]).real, complex(p[key]).imag) for p in probs])
+
+    def ptrs(key):
+        return VP(*[(p[key].value if isinstance(p[key], ctypes.c_void_p) else p[key]) for p in probs])
+
+    return dict(ta=ints("ta", _op), tb=ints("tb", _op), m=ints("m"), n=ints("n"), k=ints("k"),
+                alpha=cx("alpha"), beta=cx("beta"), A=ptrs("A"), lda=ints("lda"), B=ptrs("B"), ldb=ints("ldb"),
+                C=ptrs("C"), ldc=ints("ldc"))
+
+
+def grouped(handle, probs, precision="fp32", check=True):
+    """aclblasCgemm3mGroupedBatched over a list of problem dicts with keys
+    ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc (A/B/C device pointers)."""
+    a = _arrays(probs)
+    prec = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
+    st = handle.L.aclblasCgemm3mGroupedBatched(handle.h, len(probs), a["ta"], a["tb"], a["m"], a["n"], a["k"],
+                                               a["alpha"], a["A"], a["lda"], a["B"], a["ldb"], a["beta"], a["C"],
+                                               a["ldc"], prec)
+    if check and st != _lib.SUCCESS:
+        raise Cgemm3mError(st, "aclblasCgemm3mGroupedBatched")
+    return st
+
+
+def grouped_workspace_size(handle, probs, precision="fp32"):
+    a = _arrays([dict(p, A=None, B=None, C=None, alpha=1, beta=0, lda=1, ldb=1, ldc=1) for p in probs])
+    prec = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
+    n_bytes = ctypes.c_size_t()
+    st = handle.L.aclblasCgemm3mGroupedBatchedGetWorkspaceSize(handle.h, len(probs), a["ta"], a["tb"], a["m"],
+                                                               a["n"], a["k"], prec, ctypes.byref(n_bytes))
+    if st != _lib.SUCCESS:
+        raise Cgemm3mError(st, "aclblasCgemm3mGroupedBatchedGetWorkspaceSize")
+    return n_bytes.value

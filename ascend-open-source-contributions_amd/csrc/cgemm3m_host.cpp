@@ -43,7 +43,7 @@ StageTimer::~StageTimer()
 }
 
 static bool is_op(aclblasOperation_t t) { return t == ACLBLAS_OP_N || t == ACLBLAS_OP_T || t == ACLBLAS_OP_C; }
-static bool is_zero(const aclblasComplex& z) { return z.real == 0.0f && z.imag == 0.0f; }
+bool is_zero(const aclblasComplex& z) { return z.real == 0.0f && z.imag == 0.0f; }
 static bool is_one(const aclblasComplex& z) { return z.real == 1.0f && z.imag == 0.0f; }
 
 // Same checks, order and status codes as ValidateCgemmParams (cgemm_host.cpp:589-634),
@@ -94,7 +94,7 @@ aclblasStatus_t validate(aclblasHandle_t h, aclblasOperation_t ta, aclblasOperat
 }
 
 // ValidateCgemmPointers (cgemm_host.cpp:687-699), called with k = 0 when alpha == 0.
-static aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op)
+aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op)
 {
     if (C == nullptr) {
         log_error(op, "C must not be nullptr for a non-empty output");
@@ -138,15 +138,13 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.path = is_one(p.beta) ? 0 : 3;
         return true;
     }
-    double unit = 0x1p-24;  // unit roundoff of the product arithmetic (correction screen)
     switch (pl.precision) {
         case ACLBLAS_CGEMM3M_FP32: pl.format = FMT_F32; pl.pieces = 1; pl.bk = 16; break;
-        case ACLBLAS_CGEMM3M_BF16X3: pl.format = FMT_BF16; pl.pieces = 2; pl.bk = 16; unit = 0x1p-16; break;
+        case ACLBLAS_CGEMM3M_BF16X3: pl.format = FMT_BF16; pl.pieces = 2; pl.bk = 16; break;
         case ACLBLAS_CGEMM3M_BF16X6: pl.format = FMT_BF16; pl.pieces = 3; pl.bk = 16; break;
-        case ACLBLAS_CGEMM3M_FP16_SCALED: pl.format = FMT_F16; pl.pieces = 1; pl.bk = 16; unit = 0x1p-11; break;
+        case ACLBLAS_CGEMM3M_FP16_SCALED: pl.format = FMT_F16; pl.pieces = 1; pl.bk = 16; break;
         default: log_error(op, "invalid precision %d", pl.precision); return false;
     }
-    (void)unit;
     pl.identity = (cfg && cfg->identity == ID_STANDARD) ? ID_STANDARD : ID_FUSED;
     if (cfg && cfg->identity != 0 && cfg->identity != ID_FUSED && cfg->identity != ID_STANDARD) {
         log_error(op, "invalid identity %d", cfg->identity);
@@ -162,22 +160,28 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         }
         tile = cfg->tile;
     } else {
-        tile = tiles_for(128) >= cus ? 128 : 64;
+        int tt = 0, tp = 0;
+        if (tuned_lookup(m, n, k, false, &tt, &tp))
+            tile = tt;
+        else
+            tile = tiles_for(128) >= cus ? 128 : 64;
     }
-    // The FP16 block-scaled kernel carries two accumulator sets and only has a 64 tile.
-    if (pl.format == FMT_F16) tile = 64;
     pl.bm = pl.bn = tile;
+    // The FP16 block-scaled kernel carries two accumulator sets: 128 x 64 or 64 x 64 tiles.
+    if (pl.format == FMT_F16) pl.bn = 64;
     pl.mPad = (int)round_up((size_t)m, 128);
     pl.nPad = (int)round_up((size_t)n, 128);
     pl.kPad = (int)round_up((size_t)k, 32);
     pl.kSteps = pl.kPad / pl.bk;
-    const long long tiles = tiles_for(tile);
+    const long long tiles = ((m + pl.bm - 1) / pl.bm) * ((n + pl.bn - 1) / pl.bn);
     if (cfg && cfg->kSplit != 0) {
         if (cfg->kSplit < 1 || cfg->kSplit > 16) {
             log_error(op, "invalid kSplit %d", cfg->kSplit);
             return false;
         }
         pl.parts = std::min(cfg->kSplit, pl.kSteps);
+    } else if (int tt = 0, tp = 0; (!cfg || cfg->tile == 0) && tuned_lookup(m, n, k, false, &tt, &tp)) {
+        pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
         // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition.
         pl.parts = 1;
@@ -200,7 +204,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     if (cfg && cfg->correction != 0)
         pl.correction = cfg->correction > 0;
     else
-        pl.correction = pl.precision == ACLBLAS_CGEMM3M_FP32 && k >= 1024;
+        pl.correction = k >= 1024;
 
     // Workspace regions.
     const size_t elem = pl.format == FMT_F32 ? 4 : 2;
@@ -306,10 +310,10 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
                 e = launch_energy(ea, s);
             }
         }
+        // Correction screen, the same for every precision: recompute outputs whose condition
+        // |alpha| * |op(A)_i|_2 |op(B)_j|_2 / |c| exceeds 2^16 (or that are not finite).
         const double amag = std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
-        double unit = 0x1p-24;
-        if (pl.precision == ACLBLAS_CGEMM3M_BF16X3) unit = 0x1p-16;
-        if (pl.precision == ACLBLAS_CGEMM3M_FP16_SCALED) unit = 0x1p-11;
+        const double unit = 0x1p-24;
         GemmArgs g{};
         g.Ap = ws + pl.offA;
         g.aPlaneBytes = pl.aPlaneBytes;
@@ -607,23 +611,4 @@ extern "C" aclblasStatus_t aclblasCgemm3mResidual(aclblasHandle_t handle, aclbla
     out[2] = mxRef > 0 ? mxAbs / mxRef : mxAbs;
     out[3] = mxRef;
     return ACLBLAS_STATUS_SUCCESS;
-}
-
-extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t, int, const aclblasOperation_t*,
-                                                        const aclblasOperation_t*, const int*, const int*,
-                                                        const int*, const aclblasComplex*,
-                                                        const aclblasComplex* const*, const int*,
-                                                        const aclblasComplex* const*, const int*,
-                                                        const aclblasComplex*, aclblasComplex* const*, const int*,
-                                                        int)
-{
-    return ACLBLAS_STATUS_NOT_SUPPORTED;
-}
-
-extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasHandle_t, int,
-                                                                        const aclblasOperation_t*,
-                                                                        const aclblasOperation_t*, const int*,
-                                                                        const int*, const int*, int, size_t*)
-{
-    return ACLBLAS_STATUS_NOT_SUPPORTED;
 }

@@ -20,15 +20,13 @@
 
 namespace c3m {
 
-__global__ __launch_bounds__(256) void merge_kernel(MergeArgs a)
+// One output (i, j) of a K-split product: partitions summed in index order, then alpha/beta.
+__device__ __forceinline__ void merge_one(const MergeArgs& a, int i, int j)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.m) return;
-    for (int j = blockIdx.y; j < a.n; j += gridDim.y) {
     const size_t o = (size_t)j * a.ldp + i;
+    const size_t plane = a.partStride;  // floats per partition: re plane then im plane
+    const size_t imOff = plane / 2;
     float re = 0.0f, im = 0.0f;
-    const size_t plane = a.partStride;  // partition stride in floats
-    const size_t imOff = plane / 2;     // re plane then im plane
     for (int p = 0; p < a.parts; ++p) {
         re += a.partial[(size_t)p * plane + o];
         im += a.partial[(size_t)p * plane + imOff + o];
@@ -45,7 +43,31 @@ __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a)
         if (!bz) reinterpret_cast<float2*>(a.shadow)[lin] = c0;
     }
     C[co] = c;
-    }
+}
+
+__global__ __launch_bounds__(256) void merge_kernel(MergeArgs a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.m) return;
+    for (int j = blockIdx.y; j < a.n; j += gridDim.y) merge_one(a, i, j);
+}
+
+// Grouped batch: blockIdx.y picks the K-split problem, x walks its m*n outputs.
+__global__ __launch_bounds__(256) void merge_grouped_kernel(const GroupMerge* items)
+{
+    const MergeArgs a = items[blockIdx.y].a;
+    const long long total = (long long)a.m * a.n;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long long)gridDim.x * 256)
+        merge_one(a, (int)(t % a.m), (int)(t / a.m));
+}
+
+hipError_t launch_merge_grouped(const GroupMerge* items, int nitems, int maxOutputs, hipStream_t s)
+{
+    int bx = (maxOutputs + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(merge_grouped_kernel, dim3(bx, nitems), dim3(256), 0, s, items);
+    return hipGetLastError();
 }
 
 static unsigned grid_y(int n) { return (unsigned)(n < 65535 ? n : 65535); }

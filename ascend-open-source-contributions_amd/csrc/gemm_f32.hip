@@ -33,14 +33,12 @@ struct F32Tile {
     static_assert(A4 % 256 == 0 && B4 % 256 == 0, "stage must split evenly over 256 threads");
 };
 
+// One output tile (tmI, tnI) over K partition `part`, start to finish; lds holds
+// 2 * F32Tile<BM, BN>::S4 float4. Used by the one-problem kernel and the grouped kernel.
 template <int BM, int BN, bool PARTIAL>
-__global__ __launch_bounds__(256, 1) void gemm3m_f32_kernel(GemmArgs g)
+__device__ __forceinline__ void f32_tile(const GemmArgs& g, int part, int tmI, int tnI, f32x4* lds)
 {
     using T = F32Tile<BM, BN>;
-    __shared__ f32x4 lds[2 * T::S4];
-
-    int part, tmI, tnI;
-    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -141,6 +139,57 @@ __global__ __launch_bounds__(256, 1) void gemm3m_f32_kernel(GemmArgs g)
     const int iBase = tmI * BM + wm * (BM / 2) + (lane & 31);
     const int jBase = tnI * BN + wn * (BN / 2) + 4 * (lane >> 5);
     store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
+}
+
+template <int BM, int BN, bool PARTIAL>
+__global__ __launch_bounds__(256, 1) void gemm3m_f32_kernel(GemmArgs g)
+{
+    __shared__ f32x4 lds[2 * F32Tile<BM, BN>::S4];
+    int part, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
+    f32_tile<BM, BN, PARTIAL>(g, part, tmI, tnI, lds);
+}
+
+// Grouped batch: a persistent grid pulls tiles of many independent problems from one
+// queue (tiles pre-sorted by decreasing cost, so the longest start first). Each tile is
+// computed start to finish by one workgroup with the same arithmetic as the one-problem
+// kernel, so the output does not depend on which workgroup took it. One launch per tile
+// size, so each size runs at its own occupancy.
+template <int BM>
+__global__ __launch_bounds__(256, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_kernel(const GroupProblem* probs,
+                                                                                  const GroupTile* tiles,
+                                                                                  int ntiles, unsigned int* queue)
+{
+    __shared__ f32x4 lds[2 * F32Tile<BM, BM>::S4];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
+        if (t >= ntiles) break;
+        const GroupTile gt = tiles[t];
+        const int p = __builtin_amdgcn_readfirstlane(gt.problem);
+        const GemmArgs g = probs[p].g;
+        const int part = __builtin_amdgcn_readfirstlane(gt.part);
+        const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
+        const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
+        if (g.parts > 1)
+            f32_tile<BM, BM, true>(g, part, tm, tn, lds);
+        else
+            f32_tile<BM, BM, false>(g, part, tm, tn, lds);
+        __syncthreads();
+    }
+}
+
+hipError_t launch_gemm_f32_grouped(const GroupProblem* probs, const GroupTile* tiles, int ntiles,
+                                   unsigned int* queue, int blocks, int tile, hipStream_t s)
+{
+    if (tile == 128)
+        hipLaunchKernelGGL(gemm3m_f32_grouped_kernel<128>, dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles, queue);
+    else
+        hipLaunchKernelGGL(gemm3m_f32_grouped_kernel<64>, dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles, queue);
+    return hipGetLastError();
 }
 
 template <int BM, int BN, bool PARTIAL>

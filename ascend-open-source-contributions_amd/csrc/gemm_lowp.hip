@@ -113,39 +113,39 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
                 if (live[u]) gbuf[u] = src[u][(size_t)(ks + 1) * step[u]];
         }
         const uint4* L = lds + buf * T::S16;
+        // All fragments of the stage first, then the MFMAs ordered term-major so that 3*TN*TM
+        // independent accumulators separate two MFMAs into the same accumulator.
+        uint4 af[3][NP][T::TM], bf[3][NP][T::TN];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            uint4 af[NP][T::TM], bf[NP][T::TN];
+        for (int q = 0; q < 3; ++q)
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
 #pragma unroll
-                for (int b = 0; b < T::TM; ++b) af[p][b] = L[((q * NP + p) * T::FA + wm * T::TM + b) * 64 + lane];
+                for (int b = 0; b < T::TM; ++b)
+                    af[q][p][b] = L[((q * NP + p) * T::FA + wm * T::TM + b) * 64 + lane];
 #pragma unroll
                 for (int a = 0; a < T::TN; ++a)
-                    bf[p][a] = L[T::A16 + ((q * NP + p) * T::FB + wn * T::TN + a) * 64 + lane];
+                    bf[q][p][a] = L[T::A16 + ((q * NP + p) * T::FB + wn * T::TN + a) * 64 + lane];
             }
+        // (piece of B, piece of A) per term, small terms first:
+        //   NP = 1: (0,0);  NP = 2: (1,0) (0,1) (0,0);
+        //   NP = 3: (1,1) (2,0) (0,2) (1,0) (0,1) (0,0)  -- pieces 0 = hi, 1 = mid, 2 = lo
+        constexpr int NT = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
+        constexpr int TB[6] = {NP == 2 ? 1 : (NP == 3 ? 1 : 0), NP == 2 ? 0 : 2, 0, 1, 0, 0};
+        constexpr int TA[6] = {NP == 2 ? 0 : (NP == 3 ? 1 : 0), NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
 #pragma unroll
-            for (int a = 0; a < T::TN; ++a)
+        for (int t = 0; t < NT; ++t)
 #pragma unroll
-                for (int b = 0; b < T::TM; ++b) {
-                    if (F16S) {
-                        blk[q][a][b] = mfma16(bf[0][a], af[0][b], blk[q][a][b], true);
-                    } else if (NP == 2) {
-                        // small terms first, then hi*hi
-                        acc[q][a][b] = mfma16(bf[1][a], af[0][b], acc[q][a][b], false);
-                        acc[q][a][b] = mfma16(bf[0][a], af[1][b], acc[q][a][b], false);
-                        acc[q][a][b] = mfma16(bf[0][a], af[0][b], acc[q][a][b], false);
-                    } else {
-                        // NP == 3: pieces 0 = hi, 1 = mid, 2 = lo; the six products down to 2^-24
-                        acc[q][a][b] = mfma16(bf[1][a], af[1][b], acc[q][a][b], false);  // mid*mid
-                        acc[q][a][b] = mfma16(bf[2][a], af[0][b], acc[q][a][b], false);  // lo*hi
-                        acc[q][a][b] = mfma16(bf[0][a], af[2][b], acc[q][a][b], false);  // hi*lo
-                        acc[q][a][b] = mfma16(bf[1][a], af[0][b], acc[q][a][b], false);  // mid*hi
-                        acc[q][a][b] = mfma16(bf[0][a], af[1][b], acc[q][a][b], false);  // hi*mid
-                        acc[q][a][b] = mfma16(bf[0][a], af[0][b], acc[q][a][b], false);  // hi*hi
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                    for (int b = 0; b < T::TM; ++b) {
+                        if (F16S)
+                            blk[q][a][b] = mfma16(bf[q][0][a], af[q][0][b], blk[q][a][b], true);
+                        else
+                            acc[q][a][b] = mfma16(bf[q][TB[t]][a], af[q][TA[t]][b], acc[q][a][b], false);
                     }
-                }
-        }
         if (F16S) {
             // End of a 128-deep k block (or of this partition): fold the block's products in.
             const int kb = (ks * 16) >> 7;
@@ -209,10 +209,12 @@ hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, hipSt
 hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, hipStream_t s)
 {
     const bool partial = g.parts > 1;
-    // Two accumulator sets (running + current k block): the 128x128 tile would not fit.
-    (void)bm;
-    (void)bn;
-    return partial ? launch_one<64, 64, 1, true, true>(g, s) : launch_one<64, 64, 1, true, false>(g, s);
+    // Two accumulator sets (running sum + current 128-deep k block): at most 128 x 64.
+    if (bm == 128 && bn == 64)
+        return partial ? launch_one<128, 64, 1, true, true>(g, s) : launch_one<128, 64, 1, true, false>(g, s);
+    if (bm == 64 && bn == 64)
+        return partial ? launch_one<64, 64, 1, true, true>(g, s) : launch_one<64, 64, 1, true, false>(g, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace c3m

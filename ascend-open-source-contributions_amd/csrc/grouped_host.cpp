@@ -1,0 +1,385 @@
+// grouped_host.cpp -- aclblasCgemm3mGroupedBatched: many independent complex GEMMs in one
+// packing launch, one persistent product launch per tile size, and (only if some problem
+// is K-split) one merge launch. No reference counterpart (the reference's batched entry is
+// the uniform strided one, ops-blas/blas/gemm/arch35/cgemm_host.cpp:740-782).
+//
+// Plans per problem come from the autotuned table (tuned_table.h) with grouped = 1, else a
+// fixed rule: 128 tiles for problems at least 256 x 256, 64 tiles otherwise, no K split
+// (the batch itself fills the CUs). The descriptor block (problem arguments, pack items,
+// tile list, merge items) is built on the host and kept on the device per content, so a
+// repeated call (or a hipGraph replay of one) launches kernels only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "host_internal.h"
+#include "launch.h"
+#include "tuned_table.h"
+
+namespace c3m {
+
+bool tuned_lookup(long long m, long long n, long long k, bool grouped, int* tile, int* parts)
+{
+    if (m <= 0 || n <= 0 || k <= 0) return false;
+    int mnl = 0, kl = 0;
+    for (unsigned long long v = (unsigned long long)(m * n); v > 1; v >>= 1) ++mnl;
+    for (unsigned long long v = (unsigned long long)k; v > 1; v >>= 1) ++kl;
+    for (const TunedEntry& e : kTunedTable) {
+        if (e.tile == 0) continue;
+        if (e.mnLog2 == mnl && e.kLog2 == kl && e.grouped == (grouped ? 1 : 0)) {
+            *tile = e.tile;
+            *parts = e.parts;
+            return true;
+        }
+    }
+    return false;
+}
+
+static uint64_t fnv1a(const void* data, size_t bytes)
+{
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < bytes; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// Device copy of a descriptor block. The first call with a given content copies it from a
+// pinned host buffer on the stream; later calls with the same content reuse the device copy
+// without any copy, so a hipGraph captured after one eager call holds kernels only. If the
+// first call is itself being captured, the copy becomes a graph node and a later eager call
+// uploads again.
+static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>& blob, hipStream_t s,
+                                     hipError_t* err)
+{
+    const uint64_t key = fnv1a(blob.data(), blob.size());
+    _aclblas_handle::Staged* hit = nullptr;
+    for (auto& st : h->staged)
+        if (st.hash == key && st.bytes == blob.size() && std::memcmp(st.host, blob.data(), blob.size()) == 0)
+            hit = &st;
+    if (hit == nullptr) {
+        _aclblas_handle::Staged st{key, blob.size(), nullptr, nullptr, false};
+        if (hipHostMalloc(&st.host, blob.size(), hipHostMallocDefault) != hipSuccess) {
+            *err = hipGetLastError();
+            return nullptr;
+        }
+        if (hipMalloc(&st.device, blob.size()) != hipSuccess) {
+            *err = hipGetLastError();
+            (void)hipHostFree(st.host);
+            return nullptr;
+        }
+        std::memcpy(st.host, blob.data(), blob.size());
+        h->staged.push_back(st);
+        hit = &h->staged.back();
+    }
+    if (!hit->uploaded) {
+        *err = hipMemcpyAsync(hit->device, hit->host, hit->bytes, hipMemcpyHostToDevice, s);
+        if (*err != hipSuccess) return nullptr;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) hit->uploaded = true;
+    }
+    return static_cast<const char*>(hit->device);
+}
+
+struct GroupedLayout {
+    std::vector<Problem> probs;     // GEMM problems (alpha != 0, k > 0, m, n > 0)
+    std::vector<Plan> plans;
+    std::vector<size_t> base;       // workspace offset of each problem's region
+    std::vector<Problem> scaleOnly; // alpha == 0 or k == 0 with beta != 1
+    size_t descBytes = 0, offQueue = 0, total = 0;
+    size_t nProb = 0, nPack = 0, nTile128 = 0, nTile64 = 0, nMerge = 0;
+    long long packBlocks = 0;
+    int maxMergeOutputs = 0;
+};
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasOperation_t* ta,
+                                    const aclblasOperation_t* tb, const int* m, const int* n, const int* k,
+                                    const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
+                                    const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
+                                    aclblasComplex* const* C, const int* ldc, bool withPointers, GroupedLayout& L,
+                                    const char* op)
+{
+    for (int i = 0; i < count; ++i) {
+        const aclblasStatus_t st =
+            validate(h, ta[i], tb[i], m[i], n[i], k[i], &alpha[i], withPointers ? lda[i] : std::max(1, ta[i] != ACLBLAS_OP_N ? k[i] : m[i]),
+                     withPointers ? ldb[i] : std::max(1, tb[i] != ACLBLAS_OP_N ? n[i] : k[i]), &beta[i],
+                     withPointers ? ldc[i] : std::max(1, m[i]), op);
+        if (st != ACLBLAS_STATUS_SUCCESS) return st;
+        if (m[i] == 0 || n[i] == 0) continue;
+        Problem p{ta[i], tb[i], m[i], n[i], k[i], alpha[i], beta[i],
+                  withPointers ? A[i] : nullptr, withPointers ? lda[i] : 1,
+                  withPointers ? B[i] : nullptr, withPointers ? ldb[i] : 1,
+                  withPointers ? C[i] : nullptr, withPointers ? ldc[i] : std::max(1, m[i])};
+        if (withPointers) {
+            const aclblasStatus_t sp = validate_pointers(is_zero(p.alpha) ? 0 : p.k, p.A, p.B, p.C, op);
+            if (sp != ACLBLAS_STATUS_SUCCESS) return sp;
+        }
+        if (p.k == 0 || is_zero(p.alpha)) {
+            if (!(p.beta.real == 1.0f && p.beta.imag == 0.0f)) L.scaleOnly.push_back(p);
+            continue;
+        }
+        aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0};
+        int tt = 0, tp = 0;
+        if (tuned_lookup(p.m, p.n, p.k, true, &tt, &tp)) {
+            cfg.tile = tt;
+            cfg.kSplit = tp;
+        } else {
+            cfg.tile = (p.m >= 256 && p.n >= 256) ? 128 : 64;
+        }
+        Plan pl;
+        if (!make_plan(p, &cfg, h->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
+        L.probs.push_back(p);
+        L.plans.push_back(pl);
+    }
+    size_t total = 0;
+    for (size_t i = 0; i < L.probs.size(); ++i) {
+        L.base.push_back(total);
+        total = align256(total + L.plans[i].total);
+        const Plan& pl = L.plans[i];
+        const long long tiles = (long long)((L.probs[i].m + pl.bm - 1) / pl.bm) * ((L.probs[i].n + pl.bn - 1) / pl.bn);
+        (pl.bm == 128 ? L.nTile128 : L.nTile64) += (size_t)(tiles * pl.parts);
+        if (pl.parts > 1) {
+            ++L.nMerge;
+            L.maxMergeOutputs = std::max<long long>(L.maxMergeOutputs, (long long)L.probs[i].m * L.probs[i].n);
+        }
+    }
+    L.nProb = L.probs.size();
+    L.nPack = 2 * L.nProb;
+    L.descBytes = align256(L.nProb * sizeof(GroupProblem)) + align256(L.nPack * sizeof(GroupPack)) +
+                  align256((L.nTile128 + L.nTile64) * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge));
+    L.offQueue = total;
+    total += 256;
+    L.total = L.nProb ? total : 0;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace c3m
+
+using namespace c3m;
+
+extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasHandle_t handle, int count,
+                                                                        const aclblasOperation_t* transa,
+                                                                        const aclblasOperation_t* transb,
+                                                                        const int* m, const int* n, const int* k,
+                                                                        int precision, size_t* bytes)
+{
+    const char* op = "aclblasCgemm3mGroupedBatchedGetWorkspaceSize";
+    if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    if (bytes == nullptr || count < 0) return ACLBLAS_STATUS_INVALID_VALUE;
+    if (count == 0) {
+        *bytes = 0;
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    if (transa == nullptr || transb == nullptr || m == nullptr || n == nullptr || k == nullptr)
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    if (precision != ACLBLAS_CGEMM3M_FP32) return ACLBLAS_STATUS_NOT_SUPPORTED;
+    if (ensure_device(handle) != ACLBLAS_STATUS_SUCCESS) handle->cus = 0;
+    std::vector<aclblasComplex> one(count, aclblasComplex{1.0f, 0.0f}), zero(count, aclblasComplex{0.0f, 0.0f});
+    GroupedLayout L;
+    const aclblasStatus_t st = build_layout(handle, count, transa, transb, m, n, k, one.data(), nullptr, nullptr,
+                                            nullptr, nullptr, zero.data(), nullptr, nullptr, false, L, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    *bytes = L.total;
+    return ACLBLAS_STATUS_SUCCESS;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count,
+                                                        const aclblasOperation_t* transa,
+                                                        const aclblasOperation_t* transb, const int* m, const int* n,
+                                                        const int* k, const aclblasComplex* alpha,
+                                                        const aclblasComplex* const* A, const int* lda,
+                                                        const aclblasComplex* const* B, const int* ldb,
+                                                        const aclblasComplex* beta, aclblasComplex* const* C,
+                                                        const int* ldc, int precision)
+{
+    const char* op = "aclblasCgemm3mGroupedBatched";
+    if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    if (count < 0) return ACLBLAS_STATUS_INVALID_VALUE;
+    if (count == 0) return ACLBLAS_STATUS_SUCCESS;
+    if (transa == nullptr || transb == nullptr || m == nullptr || n == nullptr || k == nullptr || alpha == nullptr ||
+        A == nullptr || lda == nullptr || B == nullptr || ldb == nullptr || beta == nullptr || C == nullptr ||
+        ldc == nullptr) {
+        log_error(op, "argument arrays must not be nullptr");
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    if (precision != ACLBLAS_CGEMM3M_FP32) {
+        log_error(op, "precision %d is not supported for grouped batches", precision);
+        return ACLBLAS_STATUS_NOT_SUPPORTED;
+    }
+    aclblasStatus_t st = ensure_device(handle);
+    if (st != ACLBLAS_STATUS_SUCCESS) {
+        // validation first, as the one-problem entry does, then report the missing device
+        GroupedLayout L0;
+        handle->cus = 0;
+        const aclblasStatus_t v = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta,
+                                               C, ldc, true, L0, op);
+        return v != ACLBLAS_STATUS_SUCCESS ? v : st;
+    }
+    GroupedLayout L;
+    st = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, L, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    hipStream_t s = handle->stream;
+    hipError_t e = hipSuccess;
+    if (L.nProb > 0) {
+        char* ws = nullptr;
+        st = acquire_workspace(handle, L.total, &ws, op);
+        if (st != ACLBLAS_STATUS_SUCCESS) return st;
+        // Descriptor block: [problems][pack items][tiles: 128s then 64s][merge items]
+        std::vector<char> blob(L.descBytes, 0);
+        GroupProblem* gp = reinterpret_cast<GroupProblem*>(blob.data());
+        const size_t offPack = align256(L.nProb * sizeof(GroupProblem));
+        GroupPack* pk = reinterpret_cast<GroupPack*>(blob.data() + offPack);
+        const size_t offTiles = offPack + align256(L.nPack * sizeof(GroupPack));
+        GroupTile* tl = reinterpret_cast<GroupTile*>(blob.data() + offTiles);
+        const size_t offMerge = offTiles + align256((L.nTile128 + L.nTile64) * sizeof(GroupTile));
+        GroupMerge* mg = reinterpret_cast<GroupMerge*>(blob.data() + offMerge);
+        struct Cost {
+            long long cost;
+            GroupTile t;
+        };
+        std::vector<Cost> t128, t64;
+        long long packBlocks = 0;
+        size_t nm = 0;
+        for (size_t i = 0; i < L.nProb; ++i) {
+            const Problem& p = L.probs[i];
+            const Plan& pl = L.plans[i];
+            char* pws = ws + L.base[i];
+            // the same argument blocks the one-problem path builds
+            PackArgs pa{}, pb{};
+            for (int role = 0; role < 2; ++role) {
+                PackArgs& a = role == 0 ? pa : pb;
+                const bool isA = role == 0;
+                const aclblasOperation_t t = isA ? p.ta : p.tb;
+                a.src = isA ? (const void*)p.A : (const void*)p.B;
+                a.ld = isA ? p.lda : p.ldb;
+                a.mn = isA ? p.m : p.n;
+                a.kk = p.k;
+                a.mnPad = isA ? pl.mPad : pl.nPad;
+                a.kPad = pl.kPad;
+                a.mnContig = isA ? (t == ACLBLAS_OP_N) : (t != ACLBLAS_OP_N);
+                a.conj = t == ACLBLAS_OP_C;
+                a.role = role;
+                a.identity = pl.identity;
+                a.format = pl.format;
+                a.pieces = pl.pieces;
+                a.planes = pws + (isA ? pl.offA : pl.offB);
+                a.planeBytes = isA ? pl.aPlaneBytes : pl.bPlaneBytes;
+                a.energyPartial = nullptr;
+                a.scaleExp = nullptr;
+                pk[2 * i + role].a = a;
+                pk[2 * i + role].firstBlock = packBlocks;
+                packBlocks += (long long)(a.mnPad / 64) * ((a.kPad + 63) / 64);
+            }
+            GemmArgs& g = gp[i].g;
+            g = GemmArgs{};
+            g.Ap = pa.planes;
+            g.aPlaneBytes = pl.aPlaneBytes;
+            g.mPad = pl.mPad;
+            g.Bp = pb.planes;
+            g.bPlaneBytes = pl.bPlaneBytes;
+            g.nPad = pl.nPad;
+            g.m = p.m;
+            g.n = p.n;
+            g.kSteps = pl.kSteps;
+            g.parts = pl.parts;
+            g.tilesM = (p.m + pl.bm - 1) / pl.bm;
+            g.tilesN = (p.n + pl.bn - 1) / pl.bn;
+            g.identity = pl.identity;
+            g.alphaR = p.alpha.real;
+            g.alphaI = p.alpha.imag;
+            g.betaR = p.beta.real;
+            g.betaI = p.beta.imag;
+            g.betaZero = is_zero(p.beta) ? 1 : 0;
+            g.C = p.C;
+            g.ldc = p.ldc;
+            g.partial = pl.parts > 1 ? reinterpret_cast<float*>(pws + pl.offPartial) : nullptr;
+            g.partStride = pl.partStride;
+            gp[i].tile = pl.bm;
+            const long long stepsPerPart = (pl.kSteps + pl.parts - 1) / pl.parts;
+            for (int part = 0; part < pl.parts; ++part)
+                for (int tn = 0; tn < g.tilesN; ++tn)
+                    for (int tm = 0; tm < g.tilesM; ++tm) {
+                        Cost c{stepsPerPart * pl.bm * pl.bn, GroupTile{(int)i, part, tm, tn}};
+                        (pl.bm == 128 ? t128 : t64).push_back(c);
+                    }
+            if (pl.parts > 1) {
+                MergeArgs& a = mg[nm++].a;
+                a = MergeArgs{};
+                a.partial = g.partial;
+                a.partStride = pl.partStride;
+                a.parts = pl.parts;
+                a.ldp = pl.mPad;
+                a.m = p.m;
+                a.n = p.n;
+                a.alphaR = g.alphaR;
+                a.alphaI = g.alphaI;
+                a.betaR = g.betaR;
+                a.betaI = g.betaI;
+                a.betaZero = g.betaZero;
+                a.C = p.C;
+                a.ldc = p.ldc;
+            }
+        }
+        // Longest tiles first; ties in (problem, part, tm, tn) order, so the list is a pure
+        // function of the arguments.
+        auto byCost = [](const Cost& a, const Cost& b) {
+            if (a.cost != b.cost) return a.cost > b.cost;
+            if (a.t.problem != b.t.problem) return a.t.problem < b.t.problem;
+            if (a.t.part != b.t.part) return a.t.part < b.t.part;
+            if (a.t.tn != b.t.tn) return a.t.tn < b.t.tn;
+            return a.t.tm < b.t.tm;
+        };
+        std::sort(t128.begin(), t128.end(), byCost);
+        std::sort(t64.begin(), t64.end(), byCost);
+        size_t ti = 0;
+        for (auto& c : t128) tl[ti++] = c.t;
+        for (auto& c : t64) tl[ti++] = c.t;
+
+        const char* dDesc = stage_descriptors(handle, blob, s, &e);
+        if (dDesc == nullptr) {
+            log_error(op, "descriptor staging failed: %s", hipGetErrorString(e));
+            return ACLBLAS_STATUS_ALLOC_FAILED;
+        }
+        unsigned int* queue = reinterpret_cast<unsigned int*>(ws + L.offQueue);
+        const GroupProblem* dProb = reinterpret_cast<const GroupProblem*>(dDesc);
+        const GroupPack* dPack = reinterpret_cast<const GroupPack*>(dDesc + offPack);
+        const GroupTile* dTiles = reinterpret_cast<const GroupTile*>(dDesc + offTiles);
+        const GroupMerge* dMerge = reinterpret_cast<const GroupMerge*>(dDesc + offMerge);
+        {
+            StageTimer t(handle, ST_PACK);
+            e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);
+        }
+        if (e == hipSuccess) {
+            StageTimer t(handle, ST_GEMM);
+            const int cus = handle->cus > 0 ? handle->cus : 256;
+            if (!t128.empty())
+                e = launch_gemm_f32_grouped(dProb, dTiles, (int)t128.size(), queue, std::min<int>((int)t128.size(), cus),
+                                            128, s);
+            if (e == hipSuccess && !t64.empty())
+                e = launch_gemm_f32_grouped(dProb, dTiles + t128.size(), (int)t64.size(), queue + 1,
+                                            std::min<int>((int)t64.size(), 2 * cus), 64, s);
+        }
+        if (e == hipSuccess && nm > 0) {
+            StageTimer t(handle, ST_MERGE);
+            e = launch_merge_grouped(dMerge, (int)nm, L.maxMergeOutputs, s);
+        }
+    }
+    for (const Problem& p : L.scaleOnly) {
+        if (e != hipSuccess) break;
+        ScaleCArgs a{p.m, p.n, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc};
+        StageTimer t(handle, ST_SCALE);
+        e = launch_scale_c(a, s);
+    }
+    if (e != hipSuccess) {
+        log_error(op, "kernel launch failed: %s", hipGetErrorString(e));
+        return ACLBLAS_STATUS_EXECUTION_FAILED;
+    }
+    if (handle->timing) handle->timedCalls++;
+    return ACLBLAS_STATUS_SUCCESS;
+}

@@ -110,6 +110,14 @@ extern "C" aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
     if (handle->device >= 0) {
         if (hipStreamSynchronize(handle->stream) != hipSuccess) st = ACLBLAS_STATUS_EXECUTION_FAILED;
         if (handle->defWs != nullptr) (void)hipFree(handle->defWs);
+        for (auto& sb : handle->staged) {
+            (void)hipHostFree(sb.host);
+            (void)hipFree(sb.device);
+        }
+        for (auto& ev : handle->timed) {
+            (void)hipEventDestroy(ev.start);
+            (void)hipEventDestroy(ev.stop);
+        }
     }
     handle->userWs = nullptr;
     delete handle;

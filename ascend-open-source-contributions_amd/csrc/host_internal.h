@@ -26,6 +26,17 @@ struct _aclblas_handle {
     };
     std::vector<Timed> timed;
     long long timedCalls = 0;
+    // Grouped-batch descriptor blocks, keyed by content: a pinned host copy and a device
+    // copy that is uploaded once. Later calls with the same problems (and hipGraph captures
+    // of them) only read the device copy. Freed by aclblasDestroy.
+    struct Staged {
+        uint64_t hash;
+        size_t bytes;
+        void* host;
+        void* device;
+        bool uploaded;
+    };
+    std::vector<Staged> staged;
 };
 
 namespace c3m {
@@ -89,5 +100,10 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
 aclblasStatus_t ensure_device(aclblasHandle_t h);
 aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, const char* op);
 aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& plan, char* ws, const char* op);
+aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op);
+bool is_zero(const aclblasComplex& z);
+
+// Autotuned plan table (tuned_table.h): tile and K partitions for a (log2 m*n, log2 k) bin.
+bool tuned_lookup(long long m, long long n, long long k, bool grouped, int* tile, int* parts);
 
 }  // namespace c3m

@@ -122,4 +122,21 @@ struct ResidualArgs {   // FP64 reference GEMM compared with the FP32 result
     double* out;        // per block: [maxRel, sumRel, countRel, maxAbsErr, maxAbsRef]
 };
 
+// Grouped batch descriptors (device arrays in the workspace).
+struct GroupProblem {
+    GemmArgs g;           // complete arguments of this problem's product kernel
+    int tile;             // 64 or 128
+    int pad_[3];
+};
+struct GroupTile {        // one output tile of one problem (and one K partition)
+    int problem, part, tm, tn;
+};
+struct GroupPack {        // one operand of one problem for the grouped pack launch
+    PackArgs a;
+    long long firstBlock; // exclusive prefix of pack workgroups
+};
+struct GroupMerge {       // one K-split problem for the grouped merge launch
+    MergeArgs a;
+};
+
 }  // namespace c3m

@@ -18,5 +18,10 @@ hipError_t launch_scale_c(const ScaleCArgs& a, hipStream_t s);
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s);
 hipError_t launch_residual(const ResidualArgs& a, int blocksM, int blocksN, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, long long words, hipStream_t s);
+hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
+                               hipStream_t s);
+hipError_t launch_gemm_f32_grouped(const GroupProblem* probs, const GroupTile* tiles, int ntiles,
+                                   unsigned int* queue, int blocks, int tile, hipStream_t s);
+hipError_t launch_merge_grouped(const GroupMerge* items, int nitems, int maxOutputs, hipStream_t s);
 
 }  // namespace c3m

@@ -147,6 +147,36 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// Grouped batch: one launch packs both operands of every problem. Workgroup b finds its
+// (problem, operand) item by binary search over the items' first-workgroup prefix; the
+// first workgroup also resets the grouped GEMM's two tile queues.
+__global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* items, int nitems, unsigned int* queue)
+{
+    __shared__ float2 tile[64][65];
+    const long long b = blockIdx.x;
+    if (b == 0 && threadIdx.x < 2) queue[threadIdx.x] = 0u;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].firstBlock <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int idx = __builtin_amdgcn_readfirstlane(lo);
+    const PackArgs p = items[idx].a;
+    const int local = (int)(b - items[idx].firstBlock);
+    const int rowTiles = p.mnPad / 64;
+    pack_tile(p, local % rowTiles, local / rowTiles, tile);
+}
+
+hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(pack_grouped_kernel, dim3((unsigned)blocks), dim3(256), 0, s, items, nitems, queue);
+    return hipGetLastError();
+}
+
 // FP16 block scale: exponent e per (plane, 128 mn, 128 k) block with max|x| * 2^-e in
 // [2^14, 2^15), so the block's largest value keeps full FP16 precision without overflow.
 __global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)

@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--ops", default="NN,CT")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
+    ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3 counter passes: 3 plain fp32 NN calls, then one device copy "
@@ -157,6 +158,10 @@ def main():
             log("row-scaled %s err %s" % (name, res))
         del As
 
+    if not args.no_grouped:
+        results["grouped_512"] = grouped_bench(h, torch, cgemm3m, dev)
+        log("grouped", results["grouped_512"])
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(A, B, C0, m, n, k, alpha, beta, args.cpu_seconds)
@@ -197,6 +202,65 @@ def main():
         "variants": results,
     }
     print(json.dumps(line))
+
+
+def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
+    """512 uneven problems (m, n in 16*{2..32}, k in 64*{1..32}, ops uniform over {N,T,C}^2,
+    alpha = 1, beta = 0): one grouped call, the same call replayed from a hipGraph, and a
+    loop of one aclblasCgemm3m call per problem. Complex TFLOP/s counts sum(8 m n k)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    probs, keep = [], []
+    flops = 0.0
+    for _ in range(count):
+        ta, tb = str(rng.choice(list("NTC"))), str(rng.choice(list("NTC")))
+        m = int(rng.integers(2, 33)) * 16
+        n = int(rng.integers(2, 33)) * 16
+        k = int(rng.integers(1, 33)) * 64
+        lda, ldb = (m if ta == "N" else k), (k if tb == "N" else n)
+        A = (torch.rand(2 * m * k, generator=gen, device=dev) * 2 - 1).view(torch.complex64)
+        B = (torch.rand(2 * k * n, generator=gen, device=dev) * 2 - 1).view(torch.complex64)
+        C = torch.zeros(m * n, dtype=torch.complex64, device=dev)
+        keep += [A, B, C]
+        probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=1.0, A=cgemm3m.torch_ptr(A), lda=lda,
+                          B=cgemm3m.torch_ptr(B), ldb=ldb, beta=0.0, C=cgemm3m.torch_ptr(C), ldc=m))
+        flops += 8.0 * m * n * k
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    def loop():
+        for p in probs:
+            cgemm3m.cgemm3m(h, p["ta"], p["tb"], p["m"], p["n"], p["k"], 1.0, p["A"], p["lda"], p["B"], p["ldb"],
+                            0.0, p["C"], p["ldc"])
+
+    t_group = timed(lambda: cgemm3m.grouped(h, probs))
+    t_loop = timed(loop)
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        h.set_stream(side)
+        cgemm3m.grouped(h, probs)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph):
+        h.set_stream(torch.cuda.current_stream())
+        cgemm3m.grouped(h, probs)
+    h.set_stream(stream)
+    t_graph = timed(graph.replay)
+    return {"problems": count, "complex_tflop": flops / 1e12, "ms_grouped": t_group, "ms_graph_replay": t_graph,
+            "ms_per_problem_loop": t_loop, "tflops_grouped": flops / (t_group * 1e-3) / 1e12,
+            "tflops_graph": flops / (t_graph * 1e-3) / 1e12, "tflops_loop": flops / (t_loop * 1e-3) / 1e12,
+            "speedup_graph_vs_loop": t_loop / t_graph}
 
 
 def cpu_baseline(A, B, C0, m, n, k, alpha, beta, seconds):

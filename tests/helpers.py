@@ -69,4 +69,5 @@ def check_against_oracle(out, gold, initial, m, n, ldc, label=""):
     assert verdict, "%s: reference verifier failed %s" % (label, stats)
     pad = np.asarray(out)[: ldc * n].reshape(n, ldc)[:, m:]
     pad0 = np.asarray(initial)[: ldc * n].reshape(n, ldc)[:, m:]
-    assert pad.tobytes() == pad0.tobytes(), "%s: padding rows were written" % label
+    same = pad.tobytes() == pad0.tobytes()
+    assert same, "%s: padding rows were written" % label

@@ -1,0 +1,138 @@
+"""Grouped batch of uneven complex GEMMs: one packing launch + persistent product launches,
+checked per problem against the Netlib oracle, for byte-identical reruns, and replayed from
+a hipGraph (torch.cuda.CUDAGraph capture on torch's capture stream)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import inputs, ptr, region, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def make_problems(count, seed, general_scalars=False):
+    rng = np.random.default_rng(seed)
+    probs, host = [], []
+    for i in range(count):
+        ta, tb = str(rng.choice(list("NTC"))), str(rng.choice(list("NTC")))
+        m = int(rng.integers(2, 33)) * 16
+        n = int(rng.integers(2, 33)) * 16
+        k = int(rng.integers(1, 33)) * 64
+        lda = (m if ta == "N" else k) + int(rng.integers(0, 3))
+        ldb = (k if tb == "N" else n) + int(rng.integers(0, 3))
+        ldc = m + int(rng.integers(0, 3))
+        alpha, beta = (complex(0.7, -0.3), complex(0.2, 0.9)) if general_scalars else (1.0, 0.0)
+        A, B, C = inputs(ta, tb, m, n, k, lda, ldb, ldc, seed * 1000 + 3 * i)
+        dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+        probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=alpha, A=ptr(dA), lda=lda, B=ptr(dB), ldb=ldb,
+                          beta=beta, C=ptr(dC), ldc=ldc))
+        host.append(dict(A=A, B=B, C=C, dA=dA, dB=dB, dC=dC))
+    return probs, host
+
+
+def check_all(probs, host, label):
+    for i, (p, hst) in enumerate(zip(probs, host)):
+        out = hst["dC"].cpu().numpy()
+        gold = oracle.cgemm(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], hst["A"], p["lda"], hst["B"],
+                            p["ldb"], p["beta"], hst["C"], p["ldc"])
+        ok, worst = oracle.reltol_check(region(out, p["m"], p["n"], p["ldc"]), region(gold, p["m"], p["n"], p["ldc"]))
+        verdict, stats = oracle.verify(out, gold, hst["C"], p["m"], p["n"], p["ldc"])
+        assert ok and verdict, (label, i, p["ta"], p["tb"], p["m"], p["n"], p["k"], worst, stats)
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_grouped_vs_oracle(gpu, general):
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(48, 7 + general, general)
+    print("grouped problems", [(p["ta"] + p["tb"], p["m"], p["n"], p["k"]) for p in probs[:8]], "...")
+    sys.stdout.flush()
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    first = [h["dC"].cpu().numpy().copy() for h in host]
+    check_all(probs, host, "grouped")
+    for h in host:
+        h["dC"].copy_(torch.from_numpy(h["C"]).cuda())
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    for i, (h, f) in enumerate(zip(host, first)):
+        same = h["dC"].cpu().numpy().tobytes() == f.tobytes()
+        assert same, "grouped rerun differs in problem %d" % i
+
+
+def test_grouped_edge_problems(gpu):
+    """Empty, k = 0 and alpha = 0 members are handled inside one grouped call."""
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(5, 21, True)
+    probs[1]["m"] = 0
+    probs[2]["k"] = 0
+    probs[2]["beta"] = complex(0.5, -0.25)
+    probs[3]["alpha"] = 0.0
+    probs[3]["beta"] = 0.0
+    keep1 = host[1]["dC"].cpu().numpy().copy()
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    same = host[1]["dC"].cpu().numpy().tobytes() == keep1.tobytes()
+    assert same
+    for i in (0, 2, 3, 4):
+        p, hst = probs[i], host[i]
+        gold = oracle.cgemm(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], hst["A"], p["lda"], hst["B"],
+                            p["ldb"], p["beta"], hst["C"], p["ldc"])
+        ok, stats = oracle.verify(hst["dC"].cpu().numpy(), gold, hst["C"], p["m"], p["n"], p["ldc"])
+        assert ok, (i, stats)
+
+
+def test_grouped_invalid_member_launches_nothing(gpu):
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(3, 31)
+    probs[2]["lda"] = 1  # invalid leading dimension in the last member
+    before = [h["dC"].cpu().numpy().copy() for h in host]
+    st = cgemm3m.grouped(gpu, probs, check=False)
+    torch.cuda.synchronize()
+    assert st == 3
+    for h, b in zip(host, before):
+        same = h["dC"].cpu().numpy().tobytes() == b.tobytes()
+        assert same
+
+
+def test_grouped_hipgraph_replay(gpu):
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(24, 41)
+    side = torch.cuda.Stream()
+    h = cgemm3m.Handle(stream=side)
+    with torch.cuda.stream(side):
+        cgemm3m.grouped(h, probs)  # eager call sizes the workspace and stages descriptors
+    torch.cuda.synchronize()
+    eager = [x["dC"].cpu().numpy().copy() for x in host]
+    check_all(probs, host, "eager")
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        h.set_stream(torch.cuda.current_stream())
+        cgemm3m.grouped(h, probs)
+    for rep in range(3):
+        for x in host:
+            x["dC"].fill_(0)  # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        bad = []
+        for i, (x, e) in enumerate(zip(host, eager)):
+            got = x["dC"].cpu().numpy()
+            if got.tobytes() != e.tobytes():
+                bad.append((i, int((got != e).sum()), bool((got == 0).all())))
+        print("replay", rep, "differing problems (index, elements, all zero):", bad[:8])
+        assert not bad, "replay %d differs in %d problems" % (rep, len(bad))
+    h.close()

@@ -32,7 +32,8 @@ def _case(h, ta, tb, m, n, k, alpha, beta, lda, ldb, ldc, seed, cfg=None, label=
     check_against_oracle(out, gold, C, m, n, ldc, label)
     if twice:
         st2, out2 = run_gpu(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cfg, dA, dB)
-        assert st2 == 0 and out2.tobytes() == out.tobytes(), "second run differs"
+        same = st2 == 0 and out2.tobytes() == out.tobytes()
+        assert same, "second run differs"
     return out, plan
 
 
@@ -131,7 +132,8 @@ def test_tall_skinny_ksplit(gpu, parts):
     assert mine["normwise"] <= 2 * ref["normwise"] and mine["mean_rel"] <= 2 * ref["mean_rel"]
     st2, out2 = run_gpu(gpu, "N", "N", m, n, k, ALPHA2, d["A"], m, d["B"], k, BETA2, d["C"], m, cfg, d["dA"],
                         d["dB"])
-    assert out2.tobytes() == out.tobytes()
+    same = out2.tobytes() == out.tobytes()
+    assert same, "second run differs"
     if parts == 0:
         check_against_oracle(out, d["gold"], d["C"], m, n, m, "tall-skinny-auto")
 
@@ -205,7 +207,8 @@ def test_empty_output_preserves_sentinel(gpu):
     sentinel = np.full(12, complex(7, -3), dtype=np.complex64)
     for m, n in ((0, 3), (3, 0)):
         st, out = run_gpu(gpu, "N", "N", m, n, 2, 1, None, 4, None, 4, 0, sentinel, 4)
-        assert st == 0 and out.tobytes() == sentinel.tobytes()
+        same = st == 0 and out.tobytes() == sentinel.tobytes()
+        assert same
 
 
 def test_segment_cancellation(gpu):

@@ -45,7 +45,8 @@ def _run(h, prec, ta, tb, m, n, k, seed, tile=0, parts=0, scale_rows=False):
     assert st == 0
     plan = h.last_plan(sync=True)
     st2, out2 = run_gpu(h, ta, tb, m, n, k, ALPHA, A, lda, B, ldb, BETA, C, m, cfg, dA, dB)
-    assert st2 == 0 and out2.tobytes() == out.tobytes(), "rerun differs"
+    same = st2 == 0 and out2.tobytes() == out.tobytes()
+    assert same, "rerun differs"
     dC0, dOut = to_dev(C), to_dev(out)
     res = cgemm3m.residual(h, ta, tb, m, n, k, ALPHA, ptr(dA), lda, ptr(dB), ldb, BETA, ptr(dC0), ptr(dOut), m)
     print("plan", plan, "err", res)
