@@ -18,6 +18,10 @@
 
 namespace c3m {
 
+// 16-byte lane vector as an ext-vector type: arrays of it stay in VGPRs (an array of the
+// HIP struct uint4 can be moved to LDS by the compiler, serialising the prefetch loads).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 template <int BM, int BN, int NP>
 struct LowTile {
     static constexpr int TM = BM / 64, TN = BN / 64;
@@ -26,9 +30,10 @@ struct LowTile {
     static constexpr int B16 = 3 * NP * FB * 64;
     static constexpr int S16 = A16 + B16;
     static constexpr int LOADS = (S16 + 255) / 256;
+    static constexpr int TAIL = S16 - 256 * (LOADS - 1);  // threads live in the last load
 };
 
-__device__ __forceinline__ f32x16 mfma16(uint4 a, uint4 b, f32x16 c, bool half)
+__device__ __forceinline__ f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c, bool half)
 {
     if (half)
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
@@ -41,7 +46,7 @@ template <int BM, int BN, int NP, bool F16S, bool PARTIAL>
 __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
 {
     using T = LowTile<BM, BN, NP>;
-    __shared__ uint4 lds[2 * T::S16];
+    __shared__ u32x4 lds[2 * T::S16];
 
     int part, tmI, tnI;
     map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
@@ -52,26 +57,27 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
     const int ks0 = (int)((long long)g.kSteps * part / g.parts);
     const int ks1 = (int)((long long)g.kSteps * (part + 1) / g.parts);
 
-    const uint4* Ap = reinterpret_cast<const uint4*>(g.Ap);
-    const uint4* Bp = reinterpret_cast<const uint4*>(g.Bp);
+    const u32x4* Ap = reinterpret_cast<const u32x4*>(g.Ap);
+    const u32x4* Bp = reinterpret_cast<const u32x4*>(g.Bp);
     const size_t aPlane = g.aPlaneBytes / 16, bPlane = g.bPlaneBytes / 16;
     const size_t aRow = (size_t)(g.mPad / 32) * 64;  // lane vectors per 16-deep k fragment
     const size_t bRow = (size_t)(g.nPad / 32) * 64;
 
-    const uint4* src[T::LOADS];
+    const u32x4* src[T::LOADS];
     size_t step[T::LOADS];
-    bool live[T::LOADS];
+    // every load but the last is full; the last covers T::TAIL threads
+    const bool tailLive = tid < T::TAIL;
 #pragma unroll
     for (int u = 0; u < T::LOADS; ++u) {
         const int e = tid + 256 * u;
-        live[u] = e < T::S16;
+        const bool live = u + 1 < T::LOADS || tailLive;
         if (e < T::A16) {
             const int row = e / (T::FA * 64);  // plane q * NP + piece
             const int w = e % (T::FA * 64);
             src[u] = Ap + row * aPlane + (size_t)tmI * T::FA * 64 + w;
             step[u] = aRow;
         } else {
-            const int eb = live[u] ? e - T::A16 : 0;
+            const int eb = live ? e - T::A16 : 0;
             const int row = eb / (T::FB * 64);
             const int w = eb % (T::FB * 64);
             src[u] = Bp + row * bPlane + (size_t)tnI * T::FB * 64 + w;
@@ -93,14 +99,14 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
                     if (F16S) blk[q][a][b][r] = 0.0f;
                 }
 
-    uint4 gbuf[T::LOADS];
+    u32x4 gbuf[T::LOADS];
     if (ks0 < ks1) {
 #pragma unroll
         for (int u = 0; u < T::LOADS; ++u)
-            if (live[u]) gbuf[u] = src[u][(size_t)ks0 * step[u]];
+            if (u + 1 < T::LOADS || tailLive) gbuf[u] = src[u][(size_t)ks0 * step[u]];
 #pragma unroll
         for (int u = 0; u < T::LOADS; ++u)
-            if (live[u]) lds[tid + 256 * u] = gbuf[u];
+            if (u + 1 < T::LOADS || tailLive) lds[tid + 256 * u] = gbuf[u];
     }
     __syncthreads();
 
@@ -110,12 +116,12 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
         if (more) {
 #pragma unroll
             for (int u = 0; u < T::LOADS; ++u)
-                if (live[u]) gbuf[u] = src[u][(size_t)(ks + 1) * step[u]];
+                if (u + 1 < T::LOADS || tailLive) gbuf[u] = src[u][(size_t)(ks + 1) * step[u]];
         }
-        const uint4* L = lds + buf * T::S16;
+        const u32x4* L = lds + buf * T::S16;
         // All fragments of the stage first, then the MFMAs ordered term-major so that 3*TN*TM
         // independent accumulators separate two MFMAs into the same accumulator.
-        uint4 af[3][NP][T::TM], bf[3][NP][T::TN];
+        u32x4 af[3][NP][T::TM], bf[3][NP][T::TN];
 #pragma unroll
         for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
         if (more) {
 #pragma unroll
             for (int u = 0; u < T::LOADS; ++u)
-                if (live[u]) lds[(buf ^ 1) * T::S16 + tid + 256 * u] = gbuf[u];
+                if (u + 1 < T::LOADS || tailLive) lds[(buf ^ 1) * T::S16 + tid + 256 * u] = gbuf[u];
         }
         __syncthreads();
     }

@@ -122,9 +122,12 @@ def test_grouped_hipgraph_replay(gpu):
     with torch.cuda.graph(graph):
         h.set_stream(torch.cuda.current_stream())
         cgemm3m.grouped(h, probs)
+    torch.cuda.synchronize()
+    failures = []
     for rep in range(3):
         for x in host:
-            x["dC"].fill_(0)  # eager work between replays
+            # eager work between replays: restore C (padding rows included, which must stay untouched)
+            x["dC"].copy_(torch.from_numpy(x["C"]).cuda())
         torch.cuda.synchronize()
         graph.replay()
         torch.cuda.synchronize()
@@ -132,7 +135,56 @@ def test_grouped_hipgraph_replay(gpu):
         for i, (x, e) in enumerate(zip(host, eager)):
             got = x["dC"].cpu().numpy()
             if got.tobytes() != e.tobytes():
-                bad.append((i, int((got != e).sum()), bool((got == 0).all())))
-        print("replay", rep, "differing problems (index, elements, all zero):", bad[:8])
-        assert not bad, "replay %d differs in %d problems" % (rep, len(bad))
+                p = probs[i]
+                reg = region(got, p["m"], p["n"], p["ldc"])
+                zero_cols = int((np.abs(reg).sum(axis=0) == 0).sum())
+                bad.append((i, p["ta"] + p["tb"], p["m"], p["n"], p["k"], int((got != e).sum()), zero_cols,
+                            float(np.abs(got - e).max())))
+        print("replay", rep, "differing (idx, ops, m, n, k, elements, zero columns, max diff):", bad[:10])
+        sys.stdout.flush()
+        failures += bad
+    # the same call eagerly again, after the graph
+    h.set_stream(side)
+    for x in host:
+        x["dC"].copy_(torch.from_numpy(x["C"]).cuda())
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        cgemm3m.grouped(h, probs)
+    torch.cuda.synchronize()
+    again = [i for i, (x, e) in enumerate(zip(host, eager)) if x["dC"].cpu().numpy().tobytes() != e.tobytes()]
+    print("eager after graph differs in", again)
+    h.close()
+    assert not again
+    assert not failures, "%d problem replays differ" % len(failures)
+
+
+def test_single_call_hipgraph_replay(gpu):
+    """aclblasCgemm3m (FP32, fused path with correction) captured once, replayed 3 times."""
+    import torch
+
+    import cgemm3m
+
+    m, n, k = 384, 256, 1536
+    A, B, C = inputs("N", "T", m, n, k, m, n, m, 777)
+    dA, dB, dC, dC0 = to_dev(A), to_dev(B), to_dev(C), to_dev(C)
+    side = torch.cuda.Stream()
+    h = cgemm3m.Handle(stream=side)
+    with torch.cuda.stream(side):
+        cgemm3m.cgemm3m(h, "N", "T", m, n, k, 1.0, ptr(dA), m, ptr(dB), n, 0.5, ptr(dC), m)
+    torch.cuda.synchronize()
+    eager = dC.cpu().numpy().copy()
+    graph = torch.cuda.CUDAGraph()
+    dC.copy_(dC0)
+    with torch.cuda.graph(graph):
+        h.set_stream(torch.cuda.current_stream())
+        cgemm3m.cgemm3m(h, "N", "T", m, n, k, 1.0, ptr(dA), m, ptr(dB), n, 0.5, ptr(dC), m)
+    for rep in range(3):
+        dC.copy_(dC0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        got = dC.cpu().numpy()
+        ndiff = int((got != eager).sum())
+        print("single replay", rep, "differing elements", ndiff)
+        assert ndiff == 0
     h.close()
