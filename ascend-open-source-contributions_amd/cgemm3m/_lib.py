@@ -23,7 +23,7 @@ PRECISION = {"fp32": 0, "bf16x3": 1, "fp16": 2, "bf16x6": 3}
 EXPORTS = [
     "aclblasCreate", "aclblasDestroy", "aclblasSetStream", "aclblasGetStream", "aclblasSetWorkspace",
     "aclblasGetVersion", "aclblasCgemm3m", "aclblasCgemm3mEx", "aclblasCgemm3mGetWorkspaceSize",
-    "aclblasCgemm3mGroupedBatched", "aclblasCgemm3mGroupedBatchedGetWorkspaceSize",
+    "aclblasCgemm3mGroupedBatched", "aclblasCgemm3mGroupedBatchedEx", "aclblasCgemm3mGroupedBatchedGetWorkspaceSize",
     "aclblasCgemm3mResidual", "aclblasCgemm3mGetLastPlan", "aclblasCgemm3mSetTiming",
     "aclblasCgemm3mGetTiming", "aclblasSetLogCallback",
 ]
@@ -66,6 +66,9 @@ def load():
     ip = ctypes.POINTER(i)
     L.aclblasCgemm3mGroupedBatched.argtypes = [vp, i, ip, ip, ip, ip, ip, cp, ctypes.POINTER(vp), ip,
                                                ctypes.POINTER(vp), ip, cp, ctypes.POINTER(vp), ip, i]
+    L.aclblasCgemm3mGroupedBatchedEx.argtypes = [vp, i, ip, ip, ip, ip, ip, cp, ctypes.POINTER(vp), ip,
+                                                 ctypes.POINTER(vp), ip, cp, ctypes.POINTER(vp), ip,
+                                                 ctypes.POINTER(Config)]
     L.aclblasCgemm3mGroupedBatchedGetWorkspaceSize.argtypes = [vp, i, ip, ip, ip, ip, ip, i,
                                                                ctypes.POINTER(sz)]
     L.aclblasCgemm3mResidual.argtypes = [vp, i, i, i, i, i, cp, vp, i, vp, i, cp, vp, vp, i,

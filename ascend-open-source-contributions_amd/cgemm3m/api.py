@@ -149,16 +149,22 @@ def _arrays(probs):
                 C=ptrs("C"), ldc=ints("ldc"))
 
 
-def grouped(handle, probs, precision="fp32", check=True):
-    """aclblasCgemm3mGroupedBatched over a list of problem dicts with keys
-    ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc (A/B/C device pointers)."""
+def grouped(handle, probs, precision="fp32", check=True, cfg=None):
+    """aclblasCgemm3mGroupedBatched (aclblasCgemm3mGroupedBatchedEx when cfg is given) over a
+    list of problem dicts with keys ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc
+    (A/B/C device pointers)."""
     a = _arrays(probs)
-    prec = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
-    st = handle.L.aclblasCgemm3mGroupedBatched(handle.h, len(probs), a["ta"], a["tb"], a["m"], a["n"], a["k"],
-                                               a["alpha"], a["A"], a["lda"], a["B"], a["ldb"], a["beta"], a["C"],
-                                               a["ldc"], prec)
+    args = [handle.h, len(probs), a["ta"], a["tb"], a["m"], a["n"], a["k"], a["alpha"], a["A"], a["lda"], a["B"],
+            a["ldb"], a["beta"], a["C"], a["ldc"]]
+    if cfg is None:
+        prec = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
+        st = handle.L.aclblasCgemm3mGroupedBatched(*(args + [prec]))
+        what = "aclblasCgemm3mGroupedBatched"
+    else:
+        st = handle.L.aclblasCgemm3mGroupedBatchedEx(*(args + [ctypes.byref(cfg)]))
+        what = "aclblasCgemm3mGroupedBatchedEx"
     if check and st != _lib.SUCCESS:
-        raise Cgemm3mError(st, "aclblasCgemm3mGroupedBatched")
+        raise Cgemm3mError(st, what)
     return st
 
 

@@ -102,8 +102,8 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
                                     const aclblasOperation_t* tb, const int* m, const int* n, const int* k,
                                     const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
                                     const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
-                                    aclblasComplex* const* C, const int* ldc, bool withPointers, GroupedLayout& L,
-                                    const char* op)
+                                    aclblasComplex* const* C, const int* ldc, bool withPointers,
+                                    const aclblasCgemm3mConfig* over, GroupedLayout& L, const char* op)
 {
     for (int i = 0; i < count; ++i) {
         const aclblasStatus_t st =
@@ -126,12 +126,16 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
         }
         aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0};
         int tt = 0, tp = 0;
-        if (tuned_lookup(p.m, p.n, p.k, true, &tt, &tp)) {
+        if (over != nullptr && over->tile != 0) {
+            cfg.tile = over->tile;
+            cfg.kSplit = over->kSplit != 0 ? over->kSplit : 1;
+        } else if (tuned_lookup(p.m, p.n, p.k, true, &tt, &tp)) {
             cfg.tile = tt;
             cfg.kSplit = tp;
         } else {
             cfg.tile = (p.m >= 256 && p.n >= 256) ? 128 : 64;
         }
+        if (over != nullptr && over->identity != 0) cfg.identity = over->identity;
         Plan pl;
         if (!make_plan(p, &cfg, h->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
         L.probs.push_back(p);
@@ -183,22 +187,19 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasH
     std::vector<aclblasComplex> one(count, aclblasComplex{1.0f, 0.0f}), zero(count, aclblasComplex{0.0f, 0.0f});
     GroupedLayout L;
     const aclblasStatus_t st = build_layout(handle, count, transa, transb, m, n, k, one.data(), nullptr, nullptr,
-                                            nullptr, nullptr, zero.data(), nullptr, nullptr, false, L, op);
+                                            nullptr, nullptr, zero.data(), nullptr, nullptr, false, nullptr, L, op);
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
     *bytes = L.total;
     return ACLBLAS_STATUS_SUCCESS;
 }
 
-extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count,
-                                                        const aclblasOperation_t* transa,
-                                                        const aclblasOperation_t* transb, const int* m, const int* n,
-                                                        const int* k, const aclblasComplex* alpha,
-                                                        const aclblasComplex* const* A, const int* lda,
-                                                        const aclblasComplex* const* B, const int* ldb,
-                                                        const aclblasComplex* beta, aclblasComplex* const* C,
-                                                        const int* ldc, int precision)
+static aclblasStatus_t grouped_run(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
+                                   const aclblasOperation_t* transb, const int* m, const int* n, const int* k,
+                                   const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
+                                   const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
+                                   aclblasComplex* const* C, const int* ldc, int precision,
+                                   const aclblasCgemm3mConfig* over, const char* op)
 {
-    const char* op = "aclblasCgemm3mGroupedBatched";
     if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     if (count < 0) return ACLBLAS_STATUS_INVALID_VALUE;
     if (count == 0) return ACLBLAS_STATUS_SUCCESS;
@@ -218,11 +219,11 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, 
         GroupedLayout L0;
         handle->cus = 0;
         const aclblasStatus_t v = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta,
-                                               C, ldc, true, L0, op);
+                                               C, ldc, true, over, L0, op);
         return v != ACLBLAS_STATUS_SUCCESS ? v : st;
     }
     GroupedLayout L;
-    st = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, L, op);
+    st = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, over, L, op);
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
     hipStream_t s = handle->stream;
     hipError_t e = hipSuccess;
@@ -382,4 +383,36 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, 
     }
     if (handle->timing) handle->timedCalls++;
     return ACLBLAS_STATUS_SUCCESS;
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count,
+                                                        const aclblasOperation_t* transa,
+                                                        const aclblasOperation_t* transb, const int* m, const int* n,
+                                                        const int* k, const aclblasComplex* alpha,
+                                                        const aclblasComplex* const* A, const int* lda,
+                                                        const aclblasComplex* const* B, const int* ldb,
+                                                        const aclblasComplex* beta, aclblasComplex* const* C,
+                                                        const int* ldc, int precision)
+{
+    return grouped_run(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, precision,
+                       nullptr, "aclblasCgemm3mGroupedBatched");
+}
+
+extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle, int count,
+                                                          const aclblasOperation_t* transa,
+                                                          const aclblasOperation_t* transb, const int* m,
+                                                          const int* n, const int* k, const aclblasComplex* alpha,
+                                                          const aclblasComplex* const* A, const int* lda,
+                                                          const aclblasComplex* const* B, const int* ldb,
+                                                          const aclblasComplex* beta, aclblasComplex* const* C,
+                                                          const int* ldc, const aclblasCgemm3mConfig* config)
+{
+    const char* op = "aclblasCgemm3mGroupedBatchedEx";
+    if (config != nullptr && ((config->tile != 0 && config->tile != 64 && config->tile != 128) ||
+                              config->kSplit < 0 || config->kSplit > 16)) {
+        log_error(op, "invalid grouped config (tile %d, kSplit %d)", config->tile, config->kSplit);
+        return ACLBLAS_STATUS_INVALID_VALUE;
+    }
+    return grouped_run(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                       config != nullptr ? config->precision : ACLBLAS_CGEMM3M_FP32, config, op);
 }

@@ -142,6 +142,16 @@ aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count, 
                                              const aclblasComplex* beta, aclblasComplex* const* C, const int* ldc,
                                              int precision);
 
+/* For the autotuner and callers that pin plans: aclblasCgemm3mGroupedBatched with one
+ * config applied to every member (precision; tile and kSplit when nonzero; identity). */
+aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
+                                               const aclblasOperation_t* transb, const int* m, const int* n,
+                                               const int* k, const aclblasComplex* alpha,
+                                               const aclblasComplex* const* A, const int* lda,
+                                               const aclblasComplex* const* B, const int* ldb,
+                                               const aclblasComplex* beta, aclblasComplex* const* C, const int* ldc,
+                                               const aclblasCgemm3mConfig* config);
+
 /* Workspace bytes aclblasCgemm3mGroupedBatched needs for these problems. */
 aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasHandle_t handle, int count,
                                                              const aclblasOperation_t* transa,
