@@ -36,7 +36,7 @@ class Complex(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("tile", ctypes.c_int), ("kSplit", ctypes.c_int),
-                ("correction", ctypes.c_int), ("identity", ctypes.c_int)]
+                ("correction", ctypes.c_int), ("identity", ctypes.c_int), ("variant", ctypes.c_int)]
 
 
 LOG_CB = ctypes.CFUNCTYPE(None, ctypes.c_char_p)
@@ -86,5 +86,5 @@ def load():
 def complex_arg(z):
     if z is None:
         return None
-    z = complex(z)
+    z = complex(*z) if isinstance(z, (tuple, list)) else complex(z)
     return ctypes.byref(Complex(z.real, z.imag))

@@ -86,9 +86,9 @@ class Handle:
         return dict(zip(keys, list(info)))
 
 
-def config(precision="fp32", tile=0, k_split=0, correction=0, identity=0):
+def config(precision="fp32", tile=0, k_split=0, correction=0, identity=0, variant=0):
     prec = _lib.PRECISION[precision] if isinstance(precision, str) else int(precision)
-    return _lib.Config(prec, tile, k_split, correction, identity)
+    return _lib.Config(prec, tile, k_split, correction, identity, variant)
 
 
 def cgemm3m(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cfg=None, check=True):

@@ -20,6 +20,10 @@
 namespace c3m {
 
 static const char* kOp = "aclblasCgemm3m";
+// Product-kernel variant used when the caller does not pin one: eight waves per 128x128
+// tile (two per SIMD) for every format that has it (measured at 8192^3 on MI355X, complex
+// TFLOP/s four vs eight waves: FP32 149.5 vs 159.5, bf16x3 336 vs 390, bf16x6 212 vs 232).
+static int default_variant(int format) { return format == FMT_F16 ? 0 : 1; }
 
 StageTimer::StageTimer(aclblasHandle_t h, int stage) : h_(h), stage_(stage)
 {
@@ -197,6 +201,11 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         return false;
     }
     pl.path = pl.parts > 1 ? 2 : 1;
+    if (cfg && (cfg->variant < 0 || cfg->variant > 2)) {
+        log_error(op, "invalid variant %d", cfg->variant);
+        return false;
+    }
+    pl.variant = (cfg && cfg->variant != 0) ? cfg->variant - 1 : default_variant(pl.format);
     if (cfg && (cfg->correction < -1 || cfg->correction > 1)) {
         log_error(op, "invalid correction flag %d", cfg->correction);
         return false;
@@ -350,9 +359,9 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         if (e == hipSuccess) {
             StageTimer t(h, ST_GEMM);
             if (pl.format == FMT_F32)
-                e = launch_gemm_f32(g, pl.bm, pl.bn, s);
+                e = launch_gemm_f32(g, pl.bm, pl.bn, pl.variant, s);
             else if (pl.format == FMT_BF16)
-                e = launch_gemm_bf16(g, pl.bm, pl.bn, pl.pieces, s);
+                e = launch_gemm_bf16(g, pl.bm, pl.bn, pl.pieces, pl.variant, s);
             else
                 e = launch_gemm_f16(g, pl.bm, pl.bn, s);
         }

@@ -1,11 +1,12 @@
 // gemm_f32.hip -- the three real products on FP32 MFMA (v_mfma_f32_32x32x2_f32), gfx950.
 //
-// One workgroup = 4 waves = one BM x BN output tile; each wave owns a (BM/2) x (BN/2)
-// quarter and keeps all THREE product accumulators for it in registers (3 x TM x TN
-// 32x32 FP32 tiles, 192 registers at 128x128). Per k stage (BK = 16) the block copies the
+// One workgroup = WM x WN waves = one BM x BN output tile; each wave owns a (BM/WM) x (BN/WN)
+// block and keeps all THREE product accumulators for it in registers (3 x TM x TN 32x32
+// FP32 tiles: 192 registers with 4 waves at 128x128, 96 with 8 waves, two per SIMD). Per k stage (BK = 16) the block copies the
 // packed fragments of the 3 A planes and 3 B planes (contiguous runs, see kernel_args.h)
-// HBM -> VGPR -> LDS with a double-buffered LDS ring and one barrier per stage; each wave
-// then reads its fragments with one 16-byte LDS read per lane and issues the MFMAs.
+// HBM -> VGPR -> LDS with a double-buffered LDS ring, register staging one stage ahead and
+// one barrier per stage; each wave reads its fragments with one 16-byte LDS read per lane
+// and issues the MFMAs.
 //
 // The MFMA "A" operand is the B fragment and the "B" operand the A fragment, so the
 // 32x32 result tile has output rows i along the lanes: every epilogue store writes 32
@@ -21,28 +22,31 @@
 
 namespace c3m {
 
-template <int BM, int BN>
+template <int BM, int BN, int WM = 2, int WN = 2>
 struct F32Tile {
-    static constexpr int TM = BM / 64, TN = BN / 64;  // 32x32 MFMA tiles per wave (2x2 waves)
+    static constexpr int THREADS = 64 * WM * WN;        // WM x WN waves
+    static constexpr int TM = BM / WM / 32, TN = BN / WN / 32;  // 32x32 MFMA tiles per wave
     static constexpr int FA = BM / 32, FB = BN / 32;  // fragments per plane per k fragment
     static constexpr int KF = 2;                      // k fragments (8 deep) per stage
     static constexpr int A4 = 3 * KF * FA * 64;       // float4 per stage, A side
     static constexpr int B4 = 3 * KF * FB * 64;
     static constexpr int S4 = A4 + B4;
-    static constexpr int LOADS = S4 / 256;            // float4 loads per thread per stage
-    static_assert(A4 % 256 == 0 && B4 % 256 == 0, "stage must split evenly over 256 threads");
+    static constexpr int LOADS = S4 / THREADS;        // float4 loads per thread per stage
+    static_assert(A4 % THREADS == 0 && B4 % THREADS == 0, "stage must split evenly over the threads");
+    static_assert(TM >= 1 && TN >= 1, "wave tile must hold a 32x32 MFMA tile");
 };
 
 // One output tile (tmI, tnI) over K partition `part`, start to finish; lds holds
-// 2 * F32Tile<BM, BN>::S4 float4. Used by the one-problem kernel and the grouped kernel.
-template <int BM, int BN, bool PARTIAL>
+// 2 * F32Tile<BM, BN, WM, WN>::S4 float4. Used by the one-problem kernel and the grouped kernel.
+template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2>
 __device__ __forceinline__ void f32_tile(const GemmArgs& g, int part, int tmI, int tnI, f32x4* lds)
 {
-    using T = F32Tile<BM, BN>;
+    using T = F32Tile<BM, BN, WM, WN>;
+    constexpr int NT = T::THREADS;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WM, wn = wave / WM;
     const int ks0 = (int)((long long)g.kSteps * part / g.parts);
     const int ks1 = (int)((long long)g.kSteps * (part + 1) / g.parts);
 
@@ -56,8 +60,8 @@ __device__ __forceinline__ void f32_tile(const GemmArgs& g, int part, int tmI, i
     const f32x4* src[T::LOADS];
 #pragma unroll
     for (int u = 0; u < T::LOADS; ++u) {
-        const int e = tid + 256 * u;
-        if (u < T::A4 / 256) {
+        const int e = tid + NT * u;
+        if (u < T::A4 / NT) {
             const int row = e / (T::FA * 64);  // (q, kfl) pair
             const int w = e % (T::FA * 64);
             const int q = row / T::KF, kfl = row % T::KF;
@@ -81,41 +85,60 @@ __device__ __forceinline__ void f32_tile(const GemmArgs& g, int part, int tmI, i
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[q][a][b][r] = 0.0f;
 
+    // Pipeline per stage s (LDS ring of two buffers, register staging one stage ahead):
+    //   1. read every fragment of stage s from LDS into registers;
+    //   2. write the staged HBM data of stage s+1 into the other LDS buffer (last read in
+    //      stage s-1, before the barrier that ended it);
+    //   3. issue the HBM loads of stage s+2 into the staging registers;
+    //   4. issue the stage's MFMAs, then the barrier, so the barrier wait and the next
+    //      stage's LDS reads overlap the MFMAs still in flight.
     f32x4 gbuf[T::LOADS];
     if (ks0 < ks1) {
 #pragma unroll
         for (int u = 0; u < T::LOADS; ++u) {
-            const size_t step = (u < T::A4 / 256) ? (size_t)T::KF * aRow4 : (size_t)T::KF * bRow4;
+            const size_t step = (u < T::A4 / NT) ? (size_t)T::KF * aRow4 : (size_t)T::KF * bRow4;
             gbuf[u] = src[u][(size_t)ks0 * step];
         }
 #pragma unroll
-        for (int u = 0; u < T::LOADS; ++u) lds[tid + 256 * u] = gbuf[u];
+        for (int u = 0; u < T::LOADS; ++u) lds[tid + NT * u] = gbuf[u];
+        if (ks0 + 1 < ks1) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u) {
+                const size_t step = (u < T::A4 / NT) ? (size_t)T::KF * aRow4 : (size_t)T::KF * bRow4;
+                gbuf[u] = src[u][(size_t)(ks0 + 1) * step];
+            }
+        }
     }
     __syncthreads();
 
     for (int ks = ks0; ks < ks1; ++ks) {
         const int buf = (ks - ks0) & 1;
-        const bool more = ks + 1 < ks1;
-        if (more) {
-#pragma unroll
-            for (int u = 0; u < T::LOADS; ++u) {
-                const size_t step = (u < T::A4 / 256) ? (size_t)T::KF * aRow4 : (size_t)T::KF * bRow4;
-                gbuf[u] = src[u][(size_t)(ks + 1) * step];
-            }
-        }
         const f32x4* L = lds + buf * T::S4;
+        f32x4 af[T::KF][3][T::TM], bf[T::KF][3][T::TN];
 #pragma unroll
-        for (int kfl = 0; kfl < T::KF; ++kfl) {
-            f32x4 af[3][T::TM], bf[3][T::TN];
+        for (int kfl = 0; kfl < T::KF; ++kfl)
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
 #pragma unroll
                 for (int b = 0; b < T::TM; ++b)
-                    af[q][b] = L[((q * T::KF + kfl) * T::FA + wm * T::TM + b) * 64 + lane];
+                    af[kfl][q][b] = L[((q * T::KF + kfl) * T::FA + wm * T::TM + b) * 64 + lane];
 #pragma unroll
                 for (int a = 0; a < T::TN; ++a)
-                    bf[q][a] = L[T::A4 + ((q * T::KF + kfl) * T::FB + wn * T::TN + a) * 64 + lane];
+                    bf[kfl][q][a] = L[T::A4 + ((q * T::KF + kfl) * T::FB + wn * T::TN + a) * 64 + lane];
             }
+        if (ks + 1 < ks1) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u) lds[(buf ^ 1) * T::S4 + tid + NT * u] = gbuf[u];
+        }
+        if (ks + 2 < ks1) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u) {
+                const size_t step = (u < T::A4 / NT) ? (size_t)T::KF * aRow4 : (size_t)T::KF * bRow4;
+                gbuf[u] = src[u][(size_t)(ks + 2) * step];
+            }
+        }
+#pragma unroll
+        for (int kfl = 0; kfl < T::KF; ++kfl)
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -124,30 +147,25 @@ __device__ __forceinline__ void f32_tile(const GemmArgs& g, int part, int tmI, i
                     for (int a = 0; a < T::TN; ++a)
 #pragma unroll
                         for (int b = 0; b < T::TM; ++b)
-                            acc[q][a][b] =
-                                __builtin_amdgcn_mfma_f32_32x32x2f32(bf[q][a][s], af[q][b][s], acc[q][a][b], 0, 0, 0);
-        }
-        if (more) {
-#pragma unroll
-            for (int u = 0; u < T::LOADS; ++u) lds[(buf ^ 1) * T::S4 + tid + 256 * u] = gbuf[u];
-        }
+                            acc[q][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(bf[kfl][q][a][s], af[kfl][q][b][s],
+                                                                                acc[q][a][b], 0, 0, 0);
         __syncthreads();
     }
 
     // Epilogue: lane holds output row i = ... + lane%32; register r holds column
     // j = ... + 8*(r/4) + 4*(lane/32) + r%4 of each 32x32 tile.
-    const int iBase = tmI * BM + wm * (BM / 2) + (lane & 31);
-    const int jBase = tnI * BN + wn * (BN / 2) + 4 * (lane >> 5);
+    const int iBase = tmI * BM + wm * (BM / WM) + (lane & 31);
+    const int jBase = tnI * BN + wn * (BN / WN) + 4 * (lane >> 5);
     store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
 }
 
-template <int BM, int BN, bool PARTIAL>
-__global__ __launch_bounds__(256, 1) void gemm3m_f32_kernel(GemmArgs g)
+template <int BM, int BN, bool PARTIAL, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_f32_kernel(GemmArgs g)
 {
-    __shared__ f32x4 lds[2 * F32Tile<BM, BN>::S4];
+    __shared__ f32x4 lds[2 * F32Tile<BM, BN, WM, WN>::S4];
     int part, tmI, tnI;
     map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
-    f32_tile<BM, BN, PARTIAL>(g, part, tmI, tnI, lds);
+    f32_tile<BM, BN, PARTIAL, WM, WN>(g, part, tmI, tnI, lds);
 }
 
 // Grouped batch: a persistent grid pulls tiles of many independent problems from one
@@ -175,9 +193,9 @@ __global__ __launch_bounds__(256, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_ker
         const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
         const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
         if (g.parts > 1)
-            f32_tile<BM, BM, true>(g, part, tm, tn, lds);
+            f32_tile<BM, BM, true, 2, 2>(g, part, tm, tn, lds);
         else
-            f32_tile<BM, BM, false>(g, part, tm, tn, lds);
+            f32_tile<BM, BM, false, 2, 2>(g, part, tm, tn, lds);
         __syncthreads();
     }
 }
@@ -192,19 +210,26 @@ hipError_t launch_gemm_f32_grouped(const GroupProblem* probs, const GroupTile* t
     return hipGetLastError();
 }
 
-template <int BM, int BN, bool PARTIAL>
+template <int BM, int BN, bool PARTIAL, int WM, int WN>
 static hipError_t launch_one(const GemmArgs& g, hipStream_t s)
 {
     const int blocks = g.tilesM * g.tilesN * g.parts;
-    hipLaunchKernelGGL((gemm3m_f32_kernel<BM, BN, PARTIAL>), dim3(blocks), dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm3m_f32_kernel<BM, BN, PARTIAL, WM, WN>), dim3(blocks), dim3(64 * WM * WN), 0, s, g);
     return hipGetLastError();
 }
 
-hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, hipStream_t s)
+// variant 0: 4 waves per 128x128 tile (64x64 per wave); 1: 8 waves (64x32 per wave, two
+// waves per SIMD, so one covers the other's stage-boundary stalls; the default).
+hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s)
 {
     const bool partial = g.parts > 1;
-    if (bm == 128 && bn == 128) return partial ? launch_one<128, 128, true>(g, s) : launch_one<128, 128, false>(g, s);
-    if (bm == 64 && bn == 64) return partial ? launch_one<64, 64, true>(g, s) : launch_one<64, 64, false>(g, s);
+    if (bm == 128 && bn == 128) {
+        if (variant == 1)
+            return partial ? launch_one<128, 128, true, 2, 4>(g, s) : launch_one<128, 128, false, 2, 4>(g, s);
+        return partial ? launch_one<128, 128, true, 2, 2>(g, s) : launch_one<128, 128, false, 2, 2>(g, s);
+    }
+    if (bm == 64 && bn == 64)
+        return partial ? launch_one<64, 64, true, 2, 2>(g, s) : launch_one<64, 64, false, 2, 2>(g, s);
     return hipErrorInvalidValue;
 }
 

@@ -22,15 +22,16 @@ namespace c3m {
 // HIP struct uint4 can be moved to LDS by the compiler, serialising the prefetch loads).
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN, int NP>
+template <int BM, int BN, int NP, int WM = 2, int WN = 2>
 struct LowTile {
-    static constexpr int TM = BM / 64, TN = BN / 64;
+    static constexpr int THREADS = 64 * WM * WN;
+    static constexpr int TM = BM / WM / 32, TN = BN / WN / 32;
     static constexpr int FA = BM / 32, FB = BN / 32;
     static constexpr int A16 = 3 * NP * FA * 64;  // 16-byte lane vectors per stage, A side
     static constexpr int B16 = 3 * NP * FB * 64;
     static constexpr int S16 = A16 + B16;
-    static constexpr int LOADS = (S16 + 255) / 256;
-    static constexpr int TAIL = S16 - 256 * (LOADS - 1);  // threads live in the last load
+    static constexpr int LOADS = (S16 + THREADS - 1) / THREADS;
+    static constexpr int TAIL = S16 - THREADS * (LOADS - 1);  // threads live in the last load
 };
 
 __device__ __forceinline__ f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c, bool half)
@@ -42,10 +43,11 @@ __device__ __forceinline__ f32x16 mfma16(u32x4 a, u32x4 b, f32x16 c, bool half)
                                                    0, 0, 0);
 }
 
-template <int BM, int BN, int NP, bool F16S, bool PARTIAL>
-__global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
+template <int BM, int BN, int NP, bool F16S, bool PARTIAL, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g)
 {
-    using T = LowTile<BM, BN, NP>;
+    using T = LowTile<BM, BN, NP, WM, WN>;
+    constexpr int NT = T::THREADS;
     __shared__ u32x4 lds[2 * T::S16];
 
     int part, tmI, tnI;
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % WM, wn = wave / WM;
     const int ks0 = (int)((long long)g.kSteps * part / g.parts);
     const int ks1 = (int)((long long)g.kSteps * (part + 1) / g.parts);
 
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
     const bool tailLive = tid < T::TAIL;
 #pragma unroll
     for (int u = 0; u < T::LOADS; ++u) {
-        const int e = tid + 256 * u;
+        const int e = tid + NT * u;
         const bool live = u + 1 < T::LOADS || tailLive;
         if (e < T::A16) {
             const int row = e / (T::FA * 64);  // plane q * NP + piece
@@ -99,6 +101,8 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
                     if (F16S) blk[q][a][b][r] = 0.0f;
                 }
 
+    // Same stage pipeline as the FP32 kernel: fragments of stage s to registers, staged
+    // stage s+1 into the other LDS buffer, HBM loads of stage s+2, MFMAs, barrier.
     u32x4 gbuf[T::LOADS];
     if (ks0 < ks1) {
 #pragma unroll
@@ -106,52 +110,90 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
             if (u + 1 < T::LOADS || tailLive) gbuf[u] = src[u][(size_t)ks0 * step[u]];
 #pragma unroll
         for (int u = 0; u < T::LOADS; ++u)
-            if (u + 1 < T::LOADS || tailLive) lds[tid + 256 * u] = gbuf[u];
+            if (u + 1 < T::LOADS || tailLive) lds[tid + NT * u] = gbuf[u];
+        if (ks0 + 1 < ks1) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u)
+                if (u + 1 < T::LOADS || tailLive) gbuf[u] = src[u][(size_t)(ks0 + 1) * step[u]];
+        }
     }
     __syncthreads();
 
     for (int ks = ks0; ks < ks1; ++ks) {
         const int buf = (ks - ks0) & 1;
         const bool more = ks + 1 < ks1;
-        if (more) {
-#pragma unroll
-            for (int u = 0; u < T::LOADS; ++u)
-                if (u + 1 < T::LOADS || tailLive) gbuf[u] = src[u][(size_t)(ks + 1) * step[u]];
-        }
         const u32x4* L = lds + buf * T::S16;
-        // All fragments of the stage first, then the MFMAs ordered term-major so that 3*TN*TM
-        // independent accumulators separate two MFMAs into the same accumulator.
-        u32x4 af[3][NP][T::TM], bf[3][NP][T::TN];
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int p = 0; p < NP; ++p) {
-#pragma unroll
-                for (int b = 0; b < T::TM; ++b)
-                    af[q][p][b] = L[((q * NP + p) * T::FA + wm * T::TM + b) * 64 + lane];
-#pragma unroll
-                for (int a = 0; a < T::TN; ++a)
-                    bf[q][p][a] = L[T::A16 + ((q * NP + p) * T::FB + wn * T::TN + a) * 64 + lane];
-            }
         // (piece of B, piece of A) per term, small terms first:
         //   NP = 1: (0,0);  NP = 2: (1,0) (0,1) (0,0);
         //   NP = 3: (1,1) (2,0) (0,2) (1,0) (0,1) (0,0)  -- pieces 0 = hi, 1 = mid, 2 = lo
-        constexpr int NT = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
+        constexpr int NTERM = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
         constexpr int TB[6] = {NP == 2 ? 1 : (NP == 3 ? 1 : 0), NP == 2 ? 0 : 2, 0, 1, 0, 0};
         constexpr int TA[6] = {NP == 2 ? 0 : (NP == 3 ? 1 : 0), NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
+        // Fragments held at once: the whole stage (term-major MFMAs, 3*TN*TM independent
+        // accumulators between two MFMAs into one accumulator), or one plane at a time when
+        // the whole stage would not fit beside the accumulators (three pieces, eight waves).
+        constexpr bool PER_PLANE = NP * 3 * (T::TM + T::TN) * 4 > 96;
+        if (more) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
+            for (int u = 0; u < T::LOADS; ++u)
+                if (u + 1 < T::LOADS || tailLive) lds[(buf ^ 1) * T::S16 + tid + NT * u] = gbuf[u];
+        }
+        if (ks + 2 < ks1) {
+#pragma unroll
+            for (int u = 0; u < T::LOADS; ++u)
+                if (u + 1 < T::LOADS || tailLive) gbuf[u] = src[u][(size_t)(ks + 2) * step[u]];
+        }
+        if constexpr (PER_PLANE) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                u32x4 af[NP][T::TM], bf[NP][T::TN];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                    for (int b = 0; b < T::TM; ++b) af[p][b] = L[((q * NP + p) * T::FA + wm * T::TM + b) * 64 + lane];
+#pragma unroll
+                    for (int a = 0; a < T::TN; ++a)
+                        bf[p][a] = L[T::A16 + ((q * NP + p) * T::FB + wn * T::TN + a) * 64 + lane];
+                }
+#pragma unroll
+                for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+                    for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                        for (int b = 0; b < T::TM; ++b) {
+                            if (F16S)
+                                blk[q][a][b] = mfma16(bf[0][a], af[0][b], blk[q][a][b], true);
+                            else
+                                acc[q][a][b] = mfma16(bf[TB[t]][a], af[TA[t]][b], acc[q][a][b], false);
+                        }
+            }
+        } else {
+            u32x4 af[3][NP][T::TM], bf[3][NP][T::TN];
 #pragma unroll
             for (int q = 0; q < 3; ++q)
 #pragma unroll
-                for (int a = 0; a < T::TN; ++a)
+                for (int p = 0; p < NP; ++p) {
 #pragma unroll
-                    for (int b = 0; b < T::TM; ++b) {
-                        if (F16S)
-                            blk[q][a][b] = mfma16(bf[q][0][a], af[q][0][b], blk[q][a][b], true);
-                        else
-                            acc[q][a][b] = mfma16(bf[q][TB[t]][a], af[q][TA[t]][b], acc[q][a][b], false);
-                    }
+                    for (int b = 0; b < T::TM; ++b)
+                        af[q][p][b] = L[((q * NP + p) * T::FA + wm * T::TM + b) * 64 + lane];
+#pragma unroll
+                    for (int a = 0; a < T::TN; ++a)
+                        bf[q][p][a] = L[T::A16 + ((q * NP + p) * T::FB + wn * T::TN + a) * 64 + lane];
+                }
+#pragma unroll
+            for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                        for (int b = 0; b < T::TM; ++b) {
+                            if (F16S)
+                                blk[q][a][b] = mfma16(bf[q][0][a], af[q][0][b], blk[q][a][b], true);
+                            else
+                                acc[q][a][b] = mfma16(bf[q][TB[t]][a], af[q][TA[t]][b], acc[q][a][b], false);
+                        }
+        }
         if (F16S) {
             // End of a 128-deep k block (or of this partition): fold the block's products in.
             const int kb = (ks * 16) >> 7;
@@ -173,42 +215,43 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_kernel(GemmArgs g)
                 }
             }
         }
-        if (more) {
-#pragma unroll
-            for (int u = 0; u < T::LOADS; ++u)
-                if (u + 1 < T::LOADS || tailLive) lds[(buf ^ 1) * T::S16 + tid + 256 * u] = gbuf[u];
-        }
         __syncthreads();
     }
 
-    const int iBase = tmI * BM + wm * (BM / 2) + (lane & 31);
-    const int jBase = tnI * BN + wn * (BN / 2) + 4 * (lane >> 5);
+    const int iBase = tmI * BM + wm * (BM / WM) + (lane & 31);
+    const int jBase = tnI * BN + wn * (BN / WN) + 4 * (lane >> 5);
     store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
 }
 
-template <int BM, int BN, int NP, bool F16S, bool PARTIAL>
+template <int BM, int BN, int NP, bool F16S, bool PARTIAL, int WM = 2, int WN = 2>
 static hipError_t launch_one(const GemmArgs& g, hipStream_t s)
 {
     const int blocks = g.tilesM * g.tilesN * g.parts;
-    hipLaunchKernelGGL((gemm3m_lowp_kernel<BM, BN, NP, F16S, PARTIAL>), dim3(blocks), dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm3m_lowp_kernel<BM, BN, NP, F16S, PARTIAL, WM, WN>), dim3(blocks), dim3(64 * WM * WN), 0,
+                       s, g);
     return hipGetLastError();
 }
 
 template <int NP>
-static hipError_t launch_bf16_np(const GemmArgs& g, int bm, int bn, hipStream_t s)
+static hipError_t launch_bf16_np(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s)
 {
     const bool partial = g.parts > 1;
-    if (bm == 128 && bn == 128)
+    if (bm == 128 && bn == 128) {
+        if (variant == 1)
+            return partial ? launch_one<128, 128, NP, false, true, 2, 4>(g, s)
+                           : launch_one<128, 128, NP, false, false, 2, 4>(g, s);
         return partial ? launch_one<128, 128, NP, false, true>(g, s) : launch_one<128, 128, NP, false, false>(g, s);
+    }
     if (bm == 64 && bn == 64)
         return partial ? launch_one<64, 64, NP, false, true>(g, s) : launch_one<64, 64, NP, false, false>(g, s);
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, hipStream_t s)
+// variant 0: 4 waves per 128x128 tile; 1: 8 waves (two per SIMD).
+hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, int variant, hipStream_t s)
 {
-    if (pieces == 2) return launch_bf16_np<2>(g, bm, bn, s);
-    if (pieces == 3) return launch_bf16_np<3>(g, bm, bn, s);
+    if (pieces == 2) return launch_bf16_np<2>(g, bm, bn, variant, s);
+    if (pieces == 3) return launch_bf16_np<3>(g, bm, bn, variant, s);
     return hipErrorInvalidValue;
 }
 

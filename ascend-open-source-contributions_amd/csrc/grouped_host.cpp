@@ -124,7 +124,7 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             if (!(p.beta.real == 1.0f && p.beta.imag == 0.0f)) L.scaleOnly.push_back(p);
             continue;
         }
-        aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0};
+        aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0, 1};
         int tt = 0, tp = 0;
         if (over != nullptr && over->tile != 0) {
             cfg.tile = over->tile;

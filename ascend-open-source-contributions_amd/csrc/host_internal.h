@@ -72,6 +72,7 @@ struct Plan {
     int parts = 1;
     int correction = 0;
     int identity = ID_FUSED;
+    int variant = 0;      // product-kernel variant: 0 four waves per tile, 1 eight waves
     int mPad = 0, nPad = 0, kPad = 0, kSteps = 0;
     size_t aPlaneBytes = 0, bPlaneBytes = 0;
     int planesA = 3, planesB = 3;  // planes * pieces per operand
