@@ -173,12 +173,12 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_f32_kernel(GemmArgs g)
 // computed start to finish by one workgroup with the same arithmetic as the one-problem
 // kernel, so the output does not depend on which workgroup took it. One launch per tile
 // size, so each size runs at its own occupancy.
-template <int BM>
-__global__ __launch_bounds__(256, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_kernel(const GroupProblem* probs,
-                                                                                  const GroupTile* tiles,
-                                                                                  int ntiles, unsigned int* queue)
+template <int BM, int WN>
+__global__ __launch_bounds__(128 * WN, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_kernel(const GroupProblem* probs,
+                                                                                       const GroupTile* tiles,
+                                                                                       int ntiles, unsigned int* queue)
 {
-    __shared__ f32x4 lds[2 * F32Tile<BM, BM>::S4];
+    __shared__ f32x4 lds[2 * F32Tile<BM, BM, 2, WN>::S4];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
@@ -193,9 +193,9 @@ __global__ __launch_bounds__(256, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_ker
         const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
         const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
         if (g.parts > 1)
-            f32_tile<BM, BM, true, 2, 2>(g, part, tm, tn, lds);
+            f32_tile<BM, BM, true, 2, WN>(g, part, tm, tn, lds);
         else
-            f32_tile<BM, BM, false, 2, 2>(g, part, tm, tn, lds);
+            f32_tile<BM, BM, false, 2, WN>(g, part, tm, tn, lds);
         __syncthreads();
     }
 }
@@ -203,10 +203,13 @@ __global__ __launch_bounds__(256, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_ker
 hipError_t launch_gemm_f32_grouped(const GroupProblem* probs, const GroupTile* tiles, int ntiles,
                                    unsigned int* queue, int blocks, int tile, hipStream_t s)
 {
+    // 128 tiles: eight waves (as the one-problem default); 64 tiles: four waves.
     if (tile == 128)
-        hipLaunchKernelGGL(gemm3m_f32_grouped_kernel<128>, dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles, queue);
+        hipLaunchKernelGGL((gemm3m_f32_grouped_kernel<128, 4>), dim3(blocks), dim3(512), 0, s, probs, tiles, ntiles,
+                           queue);
     else
-        hipLaunchKernelGGL(gemm3m_f32_grouped_kernel<64>, dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles, queue);
+        hipLaunchKernelGGL((gemm3m_f32_grouped_kernel<64, 2>), dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles,
+                           queue);
     return hipGetLastError();
 }
 

@@ -128,3 +128,17 @@ def test_workspace_size_query(h):
     bf = cgemm3m.workspace_size(h, "N", "N", 8192, 8192, 8192, 1, 0, cgemm3m.config("bf16x3"))
     assert bf >= 6 * 8192 * 8192 * 4
     assert cgemm3m.workspace_size(h, "N", "N", 0, 5, 5, 1, 0) == 0
+
+
+def test_header_is_plain_c99(tmp_path):
+    """include/aclblas_cgemm3m.h compiles as strict C99 (a C caller needs nothing else)."""
+    import subprocess
+
+    src = tmp_path / "h.c"
+    src.write_text('#include "aclblas_cgemm3m.h"\nint main(void) { aclblasCgemm3mConfig c = {0}; (void)c; '
+                   'return (int)sizeof(aclblasComplex) - 8; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), str(src), "-o", str(tmp_path / "h")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert subprocess.run([str(tmp_path / "h")]).returncode == 0
