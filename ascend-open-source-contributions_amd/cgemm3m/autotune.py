@@ -118,7 +118,7 @@ def write_table(entries, path=TABLE):
         "namespace c3m {",
         "",
         "struct TunedEntry {",
-        "    signed char mnLog2, kLog2, grouped, tile, parts;",
+        "    short mnLog2, kLog2, grouped, tile, parts;",
         "};",
         "",
         "static const TunedEntry kTunedTable[] = {",

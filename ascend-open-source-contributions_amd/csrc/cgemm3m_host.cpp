@@ -20,10 +20,15 @@
 namespace c3m {
 
 static const char* kOp = "aclblasCgemm3m";
-// Product-kernel variant used when the caller does not pin one: eight waves per 128x128
-// tile (two per SIMD) for every format that has it (measured at 8192^3 on MI355X, complex
-// TFLOP/s four vs eight waves: FP32 149.5 vs 159.5, bf16x3 336 vs 390, bf16x6 212 vs 232).
-static int default_variant(int format) { return format == FMT_F16 ? 0 : 1; }
+// Product-kernel variant used when the caller does not pin one. FP32 128x128 tiles: the
+// LDS-free four-wave kernel (products kernel at 8192^3 on MI355X, complex TFLOP/s: 177 vs
+// 162 for eight waves through LDS); bf16 formats: eight waves per 128x128 tile (bf16x3 390
+// vs 336 for four waves, bf16x6 232 vs 212); FP16 and 64x64 tiles have one form.
+static int default_variant(int format, int bm)
+{
+    if (format == FMT_F32) return bm == 128 ? 2 : 0;
+    return format == FMT_F16 ? 0 : 1;
+}
 
 StageTimer::StageTimer(aclblasHandle_t h, int stage) : h_(h), stage_(stage)
 {
@@ -164,8 +169,9 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         }
         tile = cfg->tile;
     } else {
+        // The tuned table holds FP32-product measurements; other formats use the occupancy rule.
         int tt = 0, tp = 0;
-        if (tuned_lookup(m, n, k, false, &tt, &tp))
+        if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp))
             tile = tt;
         else
             tile = tiles_for(128) >= cus ? 128 : 64;
@@ -184,7 +190,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
             return false;
         }
         pl.parts = std::min(cfg->kSplit, pl.kSteps);
-    } else if (int tt = 0, tp = 0; (!cfg || cfg->tile == 0) && tuned_lookup(m, n, k, false, &tt, &tp)) {
+    } else if (int tt = 0, tp = 0;
+               pl.format == FMT_F32 && (!cfg || cfg->tile == 0) && tuned_lookup(m, n, k, false, &tt, &tp)) {
         pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
         // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition.
@@ -201,11 +208,12 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         return false;
     }
     pl.path = pl.parts > 1 ? 2 : 1;
-    if (cfg && (cfg->variant < 0 || cfg->variant > 2)) {
-        log_error(op, "invalid variant %d", cfg->variant);
+    if (cfg && (cfg->variant < 0 || cfg->variant > 3 ||
+                (cfg->variant == 3 && (pl.format != FMT_F32 || pl.bm != 128)))) {
+        log_error(op, "invalid variant %d for this precision and tile", cfg->variant);
         return false;
     }
-    pl.variant = (cfg && cfg->variant != 0) ? cfg->variant - 1 : default_variant(pl.format);
+    pl.variant = (cfg && cfg->variant != 0) ? cfg->variant - 1 : default_variant(pl.format, pl.bm);
     if (cfg && (cfg->correction < -1 || cfg->correction > 1)) {
         log_error(op, "invalid correction flag %d", cfg->correction);
         return false;
@@ -319,10 +327,13 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
                 e = launch_energy(ea, s);
             }
         }
-        // Correction screen, the same for every precision: recompute outputs whose condition
-        // |alpha| * |op(A)_i|_2 |op(B)_j|_2 / |c| exceeds 2^16 (or that are not finite).
+        // Correction screen: recompute outputs whose condition |alpha| |op(A)_i|_2 |op(B)_j|_2 / |c|
+        // exceeds 2^8 / unit (or that are not finite). unit = 2^-24 (FP32 rounding) for FP32,
+        // bf16x6 and FP16; bf16x3 drops the lo*lo term (relative size up to 2^-18 per term), so
+        // its screen is four times wider (2^-22), which keeps its worst per-element relative
+        // error near the FP32 path's. FP16 block-scaled products bound the normwise error only.
         const double amag = std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
-        const double unit = 0x1p-24;
+        const double unit = pl.precision == ACLBLAS_CGEMM3M_BF16X3 ? 0x1p-22 : 0x1p-24;
         GemmArgs g{};
         g.Ap = ws + pl.offA;
         g.aPlaneBytes = pl.aPlaneBytes;

@@ -168,6 +168,106 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_f32_kernel(GemmArgs g)
     f32_tile<BM, BN, PARTIAL, WM, WN>(g, part, tmI, tnI, lds);
 }
 
+// LDS-free form of f32_tile. The packed planes are already in MFMA fragment order, so each
+// wave loads its own fragments straight into registers (one 1 KiB coalesced load per
+// fragment) and no data passes through LDS: no barriers, and the waves of a workgroup run
+// independently. Two register sets alternate: the loads of stage s+1 are issued before the
+// MFMAs of stage s, so their latency hides behind a whole stage of MFMAs. Fragments that
+// several waves of the workgroup share are read by each of them (served by the CU's vector
+// cache and L2). Same MFMA order per accumulator as f32_tile, so the results are identical.
+template <int BM, int BN, int WM, int WN>
+struct DirectSet {
+    using T = F32Tile<BM, BN, WM, WN>;
+    f32x4 a[T::KF][3][T::TM], b[T::KF][3][T::TN];
+};
+
+template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2>
+__device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int tmI, int tnI)
+{
+    using T = F32Tile<BM, BN, WM, WN>;
+    using Set = DirectSet<BM, BN, WM, WN>;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave % WM, wn = wave / WM;
+    const int ks0 = (int)((long long)g.kSteps * part / g.parts);
+    const int ks1 = (int)((long long)g.kSteps * (part + 1) / g.parts);
+
+    // Wave-uniform byte addresses (scalar registers) plus one 32-bit lane offset, so every
+    // load is a scalar base + vector offset form and addressing costs almost no VGPRs.
+    const size_t aRowB = (size_t)(g.mPad / 32) * 1024;  // bytes per k fragment, all of A's mn
+    const size_t bRowB = (size_t)(g.nPad / 32) * 1024;
+    const char* aW = reinterpret_cast<const char*>(g.Ap) + (size_t)(tmI * T::FA + wm * T::TM) * 1024;
+    const char* bW = reinterpret_cast<const char*>(g.Bp) + (size_t)(tnI * T::FB + wn * T::TN) * 1024;
+    const unsigned laneB = (unsigned)lane * 16u;
+
+    auto load = [&](Set& S, int ks) {
+#pragma unroll
+        for (int kfl = 0; kfl < T::KF; ++kfl)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const char* a = aW + (size_t)q * g.aPlaneBytes + (size_t)(ks * T::KF + kfl) * aRowB;
+                const char* b = bW + (size_t)q * g.bPlaneBytes + (size_t)(ks * T::KF + kfl) * bRowB;
+#pragma unroll
+                for (int i = 0; i < T::TM; ++i)
+                    S.a[kfl][q][i] = *reinterpret_cast<const f32x4*>(a + i * 1024 + laneB);
+#pragma unroll
+                for (int j = 0; j < T::TN; ++j)
+                    S.b[kfl][q][j] = *reinterpret_cast<const f32x4*>(b + j * 1024 + laneB);
+            }
+    };
+
+    f32x16 acc[3][T::TN][T::TM];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+            for (int b = 0; b < T::TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[q][a][b][r] = 0.0f;
+
+    auto mfma = [&](const Set& S) {
+#pragma unroll
+        for (int kfl = 0; kfl < T::KF; ++kfl)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                        for (int b = 0; b < T::TM; ++b)
+                            acc[q][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(S.b[kfl][q][a][s], S.a[kfl][q][b][s],
+                                                                                acc[q][a][b], 0, 0, 0);
+    };
+
+    // Stages in pairs (set 0, set 1) with one exit, so the accumulators keep their registers;
+    // an odd last stage is already loaded into set 0.
+    Set s0, s1;
+    const int nst = ks1 - ks0;
+    if (nst > 0) load(s0, ks0);
+    int st = 0;
+    for (; st + 1 < nst; st += 2) {
+        load(s1, ks0 + st + 1);
+        mfma(s0);
+        if (st + 2 < nst) load(s0, ks0 + st + 2);
+        mfma(s1);
+    }
+    if (st < nst) mfma(s0);
+
+    const int iBase = tmI * BM + wm * (BM / WM) + (lane & 31);
+    const int jBase = tnI * BN + wn * (BN / WN) + 4 * (lane >> 5);
+    store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
+}
+
+template <int BM, int BN, bool PARTIAL, int WM, int WN, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void gemm3m_f32_direct_kernel(GemmArgs g)
+{
+    int part, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
+    f32_tile_direct<BM, BN, PARTIAL, WM, WN>(g, part, tmI, tnI);
+}
+
 // Grouped batch: a persistent grid pulls tiles of many independent problems from one
 // queue (tiles pre-sorted by decreasing cost, so the longest start first). Each tile is
 // computed start to finish by one workgroup with the same arithmetic as the one-problem
@@ -200,12 +300,38 @@ __global__ __launch_bounds__(128 * WN, BM == 128 ? 1 : 2) void gemm3m_f32_groupe
     }
 }
 
+// Grouped 128x128 tiles with the LDS-free four-wave tile (the one-problem default).
+__global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_direct_kernel(const GroupProblem* probs,
+                                                                          const GroupTile* tiles, int ntiles,
+                                                                          unsigned int* queue)
+{
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
+        if (t >= ntiles) break;
+        const GroupTile gt = tiles[t];
+        const int p = __builtin_amdgcn_readfirstlane(gt.problem);
+        const GemmArgs g = probs[p].g;
+        const int part = __builtin_amdgcn_readfirstlane(gt.part);
+        const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
+        const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
+        if (g.parts > 1)
+            f32_tile_direct<128, 128, true, 2, 2>(g, part, tm, tn);
+        else
+            f32_tile_direct<128, 128, false, 2, 2>(g, part, tm, tn);
+    }
+}
+
 hipError_t launch_gemm_f32_grouped(const GroupProblem* probs, const GroupTile* tiles, int ntiles,
                                    unsigned int* queue, int blocks, int tile, hipStream_t s)
 {
-    // 128 tiles: eight waves (as the one-problem default); 64 tiles: four waves.
+    // 128 tiles: the LDS-free four-wave tile (as the one-problem default); 64 tiles: four
+    // waves through LDS.
     if (tile == 128)
-        hipLaunchKernelGGL((gemm3m_f32_grouped_kernel<128, 4>), dim3(blocks), dim3(512), 0, s, probs, tiles, ntiles,
+        hipLaunchKernelGGL(gemm3m_f32_grouped_direct_kernel, dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles,
                            queue);
     else
         hipLaunchKernelGGL((gemm3m_f32_grouped_kernel<64, 2>), dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles,
@@ -221,11 +347,25 @@ static hipError_t launch_one(const GemmArgs& g, hipStream_t s)
     return hipGetLastError();
 }
 
-// variant 0: 4 waves per 128x128 tile (64x64 per wave); 1: 8 waves (64x32 per wave, two
-// waves per SIMD, so one covers the other's stage-boundary stalls; the default).
+template <int BM, int BN, bool PARTIAL, int WM, int WN, int OCC>
+static hipError_t launch_direct(const GemmArgs& g, hipStream_t s)
+{
+    const int blocks = g.tilesM * g.tilesN * g.parts;
+    hipLaunchKernelGGL((gemm3m_f32_direct_kernel<BM, BN, PARTIAL, WM, WN, OCC>), dim3(blocks), dim3(64 * WM * WN), 0,
+                       s, g);
+    return hipGetLastError();
+}
+
+// 128x128 tiles: variant 0: 4 waves through LDS (64x64 per wave); 1: 8 waves through LDS
+// (64x32 per wave, two per SIMD); 2: LDS-free, 4 waves (the default). Measured at 8192^3 on
+// MI355X, products kernel alone, complex TFLOP/s: 151.7, 162.5, 177.2. The LDS-free form
+// with 8 waves (92) or 64x64 tiles (85, against 150 through LDS) loses: its wave tiles are
+// too small for the operands it re-reads from the caches, so it is not built.
 hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s)
 {
     const bool partial = g.parts > 1;
+    if (bm == 128 && bn == 128 && variant == 2)
+        return partial ? launch_direct<128, 128, true, 2, 2, 1>(g, s) : launch_direct<128, 128, false, 2, 2, 1>(g, s);
     if (bm == 128 && bn == 128) {
         if (variant == 1)
             return partial ? launch_one<128, 128, true, 2, 4>(g, s) : launch_one<128, 128, false, 2, 4>(g, s);

@@ -27,32 +27,49 @@ __device__ __forceinline__ float plane_value(int role, int identity, int q, floa
     return q == 0 ? re : (q == 1 ? im - re : re + im);            // B: (Br, Bi-Br, Br+Bi)
 }
 
-__device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, float2 (*tile)[65])
+__device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, float2 (*tile)[65], double* esum)
 {
     const int tid = threadIdx.x;
     const int r0 = rt * 64, c0 = ct * 64;
     const float2* src = reinterpret_cast<const float2*>(a.src);
-    for (int e = tid; e < 4096; e += 256) {
+    // All 16 loads of this thread are issued before any is used: out-of-range elements read
+    // a clamped in-range address and are replaced by zero, so there is no branch around a
+    // load (pack only runs with mn, kk >= 1).
+    float2 v[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int e = tid + 256 * t;
         const int fast = e & 63, slow = e >> 6;
-        const int rl = a.mnContig ? fast : slow;
-        const int cl = a.mnContig ? slow : fast;
-        const int r = r0 + rl, c = c0 + cl;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (r < a.mn && c < a.kk) {
-            v = a.mnContig ? src[(size_t)c * a.ld + r] : src[(size_t)r * a.ld + c];
-            if (a.conj) v.y = -v.y;
-        }
-        tile[rl][cl] = v;
+        const int r = r0 + (a.mnContig ? fast : slow);
+        const int c = c0 + (a.mnContig ? slow : fast);
+        const int rr = min(r, a.mn - 1), cc = min(c, a.kk - 1);
+        v[t] = a.mnContig ? src[(size_t)cc * a.ld + rr] : src[(size_t)rr * a.ld + cc];
+        const bool in = r < a.mn && c < a.kk;
+        v[t].x = in ? v[t].x : 0.0f;
+        v[t].y = in ? (a.conj ? -v[t].y : v[t].y) : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int e = tid + 256 * t;
+        const int fast = e & 63, slow = e >> 6;
+        tile[a.mnContig ? fast : slow][a.mnContig ? slow : fast] = v[t];
     }
     __syncthreads();
 
-    if (a.energyPartial != nullptr && tid < 64) {
-        double s = 0.0;
-        for (int cl = 0; cl < 64; ++cl) {
-            const float2 v = tile[tid][cl];
-            s += (double)v.x * v.x + (double)v.y * v.y;
+    if (a.energyPartial != nullptr) {
+        // row = tid % 64 sums its quarter of the tile's 64 k in order, then the four quarter
+        // sums are added in quarter order: a fixed order, so the result is reproducible.
+        const int row = tid & 63, qtr = tid >> 6;
+        double sq = 0.0;
+        for (int cl = qtr * 16; cl < qtr * 16 + 16; ++cl) {
+            const float2 w = tile[row][cl];
+            sq += (double)w.x * w.x + (double)w.y * w.y;
         }
-        a.energyPartial[(size_t)ct * a.mnPad + r0 + tid] = s;
+        esum[qtr * 64 + row] = sq;
+        __syncthreads();
+        if (tid < 64)
+            a.energyPartial[(size_t)ct * a.mnPad + r0 + tid] =
+                ((esum[tid] + esum[64 + tid]) + esum[128 + tid]) + esum[192 + tid];
     }
 
     const size_t mnFrags = (size_t)(a.mnPad / 32);
@@ -70,8 +87,8 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
                 f32x4 out;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const float2 v = tile[rl][cl + s];
-                    out[s] = plane_value(a.role, a.identity, q, v.x, v.y);
+                    const float2 w = tile[rl][cl + s];
+                    out[s] = plane_value(a.role, a.identity, q, w.x, w.y);
                 }
                 reinterpret_cast<f32x4*>(static_cast<char*>(a.planes) + q * a.planeBytes)[idx] = out;
             }
@@ -90,8 +107,8 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
                 float x[8];
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {
-                    const float2 v = tile[rl][cl + s];
-                    x[s] = plane_value(a.role, a.identity, q, v.x, v.y);
+                    const float2 w = tile[rl][cl + s];
+                    x[s] = plane_value(a.role, a.identity, q, w.x, w.y);
                 }
                 if (a.format == FMT_BF16) {
                     for (int p = 0; p < a.pieces; ++p) {
@@ -125,11 +142,12 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs a, PackArgs b, int blocksA)
 {
     __shared__ float2 tile[64][65];
+    __shared__ double esum[256];
     const bool isA = (int)blockIdx.x < blocksA;
     const PackArgs& p = isA ? a : b;
     const int local = isA ? blockIdx.x : blockIdx.x - blocksA;
     const int rowTiles = p.mnPad / 64;
-    pack_tile(p, local % rowTiles, local / rowTiles, tile);
+    pack_tile(p, local % rowTiles, local / rowTiles, tile, esum);
 }
 
 static int pack_blocks(const PackArgs& a) { return (a.mnPad / 64) * ((a.kPad + 63) / 64); }
@@ -153,6 +171,7 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t s)
 __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* items, int nitems, unsigned int* queue)
 {
     __shared__ float2 tile[64][65];
+    __shared__ double esum[256];
     const long long b = blockIdx.x;
     if (b == 0 && threadIdx.x < 2) queue[threadIdx.x] = 0u;
     int lo = 0, hi = nitems - 1;
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* item
     const PackArgs p = items[idx].a;
     const int local = (int)(b - items[idx].firstBlock);
     const int rowTiles = p.mnPad / 64;
-    pack_tile(p, local % rowTiles, local / rowTiles, tile);
+    pack_tile(p, local % rowTiles, local / rowTiles, tile, esum);
 }
 
 hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,

@@ -113,7 +113,9 @@ typedef struct {
     int kSplit;     /* 0 auto, else 1..16 K partitions reduced in fixed order */
     int correction; /* 0 auto, 1 force on, -1 force off (FP32 only) */
     int identity;   /* 0 auto (fused identity), 1 fused, 2 standard 3M identity */
-    int variant;    /* 0 auto, 1: four waves per 128x128 tile, 2: eight waves (two per SIMD) */
+    int variant;    /* 0 auto; 128x128 tiles: 1 four waves through LDS, 2 eight waves through
+                       LDS (two per SIMD), 3 four waves loading fragments straight into
+                       registers, no LDS (FP32 only) */
 } aclblasCgemm3mConfig;
 
 /* For callers that choose the arithmetic or pin the plan (bench, tests, autotuner, the

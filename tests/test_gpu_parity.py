@@ -151,6 +151,30 @@ def test_identity_and_tile(gpu, identity, tile):
           cfg=cfg, label="id%d-t%d" % (identity, tile))
 
 
+@pytest.mark.parametrize("prec,variants", [("fp32", (1, 2, 3)), ("bf16x3", (1, 2))])
+@pytest.mark.parametrize("parts", [1, 3])
+def test_kernel_variants_identical(gpu, prec, variants, parts):
+    """The 128x128 product kernels (four or eight waves through LDS, four waves loading
+    fragments straight into registers) issue the same MFMAs per accumulator in the same
+    order, so their outputs are byte-identical; the first also passes the oracle check."""
+    import cgemm3m
+
+    m, n, k = 1000, 700, 1100
+    outs = []
+    for v in variants:
+        cfg = cgemm3m.config(prec, tile=128, k_split=parts, variant=v)
+        if v == variants[0] and prec == "fp32":
+            out, _ = _case(gpu, "C", "N", m, n, k, ALPHA2, BETA2, k + 3, k + 1, m + 2, 20260550 + parts, cfg=cfg,
+                           label="variant%d" % v)
+        else:
+            A, B, C = inputs("C", "N", m, n, k, k + 3, k + 1, m + 2, 20260550 + parts)
+            st, out = run_gpu(gpu, "C", "N", m, n, k, ALPHA2, A, k + 3, B, k + 1, BETA2, C, m + 2, cfg)
+            assert st == 0
+        outs.append(out)
+    for v, o in zip(variants[1:], outs[1:]):
+        assert o.tobytes() == outs[0].tobytes(), "variant %d differs from variant %d" % (v, variants[0])
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_small_ragged(gpu, seed):
     rng = np.random.default_rng(seed)
