@@ -178,7 +178,7 @@ def main():
     if "gemm_tflops" in main_r:
         roof = {"bound": "mfma", "achieved": round(main_r["gemm_tflops"], 2), "peak": PEAK["fp32"],
                 "unit": "TFLOP/s", "frac": round(main_r["gemm_frac_of_peak"], 4), "traffic": traffic,
-                "kernel": "gemm3m_f32_kernel<128,128> (6mnk FP32 MFMA flops per launch)"}
+                "kernel": "gemm3m_f32_direct_kernel<128,128> (6mnk FP32 MFMA flops per launch)"}
     bf = results.get("bf16x3_NN", {})
     line = {
         "metric": METRIC,
