@@ -5,8 +5,8 @@ binds it (ctypes) and adds the torch autograd wrapper. torch supplies device mem
 streams.
 """
 from . import _lib
-from .api import (Cgemm3mError, Handle, cgemm3m, config, grouped, grouped_workspace_size, residual, torch_ptr,
-                  workspace_size)
+from .api import (Cgemm3mError, Handle, cgemm, cgemm3m, config, grouped, grouped_workspace_size, residual,
+                  strided_batched, torch_ptr, workspace_size)
 
-__all__ = ["Cgemm3mError", "Handle", "cgemm3m", "config", "grouped", "grouped_workspace_size", "residual",
-           "torch_ptr", "workspace_size", "_lib"]
+__all__ = ["Cgemm3mError", "Handle", "cgemm", "cgemm3m", "config", "grouped", "grouped_workspace_size", "residual",
+           "strided_batched", "torch_ptr", "workspace_size", "_lib"]

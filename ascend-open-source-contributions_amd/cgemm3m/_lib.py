@@ -15,14 +15,15 @@ STATUS = {
     5: "EXECUTION_FAILED", 6: "INTERNAL_ERROR", 7: "NOT_SUPPORTED", 8: "ARCH_MISMATCH",
     9: "HANDLE_IS_NULLPTR", 10: "INVALID_ENUM", 11: "UNKNOWN",
 }
-SUCCESS, INVALID_VALUE, EXECUTION_FAILED, NOT_SUPPORTED, HANDLE_IS_NULLPTR = 0, 3, 5, 7, 9
+SUCCESS, INVALID_VALUE, EXECUTION_FAILED, NOT_SUPPORTED, HANDLE_IS_NULLPTR, INVALID_ENUM = 0, 3, 5, 7, 9, 10
 OP = {"N": 111, "T": 112, "C": 113}
 PRECISION = {"fp32": 0, "bf16x3": 1, "fp16": 2, "bf16x6": 3}
 
 # Every symbol include/aclblas_cgemm3m.h declares.
 EXPORTS = [
     "aclblasCreate", "aclblasDestroy", "aclblasSetStream", "aclblasGetStream", "aclblasSetWorkspace",
-    "aclblasGetVersion", "aclblasCgemm3m", "aclblasCgemm3mEx", "aclblasCgemm3mGetWorkspaceSize",
+    "aclblasGetVersion", "aclblasCgemm3m", "aclblasCgemm", "aclblasCgemm3mStridedBatched", "aclblasCgemm3mEx",
+    "aclblasCgemm3mGetWorkspaceSize",
     "aclblasCgemm3mGroupedBatched", "aclblasCgemm3mGroupedBatchedEx", "aclblasCgemm3mGroupedBatchedGetWorkspaceSize",
     "aclblasCgemm3mResidual", "aclblasCgemm3mGetLastPlan", "aclblasCgemm3mSetTiming",
     "aclblasCgemm3mGetTiming", "aclblasSetLogCallback",
@@ -60,7 +61,10 @@ def load():
     L.aclblasGetVersion.argtypes = [vp, ctypes.POINTER(i)]
     gemm = [vp, i, i, i, i, i, cp, vp, i, vp, i, cp, vp, i]
     L.aclblasCgemm3m.argtypes = gemm
+    L.aclblasCgemm.argtypes = gemm
     L.aclblasCgemm3mEx.argtypes = gemm + [ctypes.POINTER(Config)]
+    i64 = ctypes.c_int64
+    L.aclblasCgemm3mStridedBatched.argtypes = [vp, i, i, i, i, i, cp, vp, i, i64, vp, i, i64, cp, vp, i, i64, i]
     L.aclblasCgemm3mGetWorkspaceSize.argtypes = [vp, i, i, i, i, i, cp, cp, ctypes.POINTER(Config),
                                                  ctypes.POINTER(sz)]
     ip = ctypes.POINTER(i)

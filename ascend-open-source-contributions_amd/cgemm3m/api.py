@@ -107,6 +107,27 @@ def cgemm3m(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc
     return st
 
 
+def cgemm(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, check=True):
+    """aclblasCgemm: the four-product complex GEMM. Returns the status code."""
+    st = handle.L.aclblasCgemm(handle.h if handle is not None else None, _op(transa), _op(transb), m, n, k,
+                               _lib.complex_arg(alpha), A, lda, B, ldb, _lib.complex_arg(beta), C, ldc)
+    if check and st != _lib.SUCCESS:
+        raise Cgemm3mError(st, "aclblasCgemm")
+    return st
+
+
+def strided_batched(handle, transa, transb, m, n, k, alpha, A, lda, stride_a, B, ldb, stride_b, beta, C, ldc,
+                    stride_c, batch, check=True):
+    """aclblasCgemm3mStridedBatched (strides in complex elements). Returns the status code."""
+    L = handle.L
+    st = L.aclblasCgemm3mStridedBatched(handle.h if handle is not None else None, _op(transa), _op(transb), m, n,
+                                        k, _lib.complex_arg(alpha), A, lda, stride_a, B, ldb, stride_b,
+                                        _lib.complex_arg(beta), C, ldc, stride_c, batch)
+    if check and st != _lib.SUCCESS:
+        raise Cgemm3mError(st, "aclblasCgemm3mStridedBatched")
+    return st
+
+
 def residual(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, Cinit, C, ldc):
     """FP64 error report of C against alpha*op(A)op(B)+beta*Cinit (synchronises)."""
     out = (ctypes.c_double * 4)()

@@ -29,6 +29,15 @@ def bin_of(m, n, k):
     return int(math.floor(math.log2(m * n))), int(math.floor(math.log2(k)))
 
 
+def choose(res, margin=0.02):
+    """Best (tile, parts) of {(tile, parts): ms}: among the candidates within `margin` of the
+    fastest, the one with the fewest K partitions (less workspace, no merge pass), then the
+    larger tile, so timing noise does not turn a tie into a K split."""
+    best = min(res.values())
+    close = [c for c, t in res.items() if t <= best * (1.0 + margin)]
+    return min(close, key=lambda c: (c[1], -c[0]))
+
+
 def _timer(torch):
     def timed(fn, reps):
         fn()
@@ -64,7 +73,7 @@ def tune_single(h, torch, shapes, reps):
                 cfg = cgemm3m.config("fp32", tile=tile, k_split=parts)
                 res[(tile, parts)] = timed(lambda: cgemm3m.cgemm3m(h, "N", "N", m, n, k, 1.0, P(A), m, P(B), k,
                                                                    0.0, P(C), m, cfg), reps)
-        best = min(res, key=res.get)
+        best = choose(res)
         print("single %5d x %5d x %6d  best tile %d parts %2d  %.3f ms  (all: %s)" % (
             m, n, k, best[0], best[1], res[best], {"%d/%d" % kk: round(v, 3) for kk, v in res.items()}),
             flush=True)
@@ -97,7 +106,7 @@ def tune_grouped(h, torch, shapes, reps, count=64):
                     continue
                 cfg = cgemm3m.config("fp32", tile=tile, k_split=parts)
                 res[(tile, parts)] = timed(lambda: cgemm3m.grouped(h, probs, cfg=cfg), reps)
-        best = min(res, key=res.get)
+        best = choose(res)
         print("grouped %4d x %4d x %5d  best tile %d parts %d  (all: %s)" % (
             m, n, k, best[0], best[1], {"%d/%d" % kk: round(v, 3) for kk, v in res.items()}), flush=True)
         out.append((bin_of(m, n, k), 1, best))

@@ -21,7 +21,7 @@ namespace c3m {
 
 static const char* kOp = "aclblasCgemm3m";
 // Product-kernel variant used when the caller does not pin one. FP32 128x128 tiles: the
-// LDS-free four-wave kernel (products kernel at 8192^3 on MI355X, complex TFLOP/s: 177 vs
+// LDS-free four-wave kernel (products kernel at 8192^3 on MI355X, complex TFLOP/s: 186 vs
 // 162 for eight waves through LDS); bf16 formats: eight waves per 128x128 tile (bf16x3 390
 // vs 336 for four waves, bf16x6 232 vs 212); FP16 and 64x64 tiles have one form.
 static int default_variant(int format, int bm)
@@ -154,9 +154,14 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         case ACLBLAS_CGEMM3M_FP16_SCALED: pl.format = FMT_F16; pl.pieces = 1; pl.bk = 16; break;
         default: log_error(op, "invalid precision %d", pl.precision); return false;
     }
-    pl.identity = (cfg && cfg->identity == ID_STANDARD) ? ID_STANDARD : ID_FUSED;
-    if (cfg && cfg->identity != 0 && cfg->identity != ID_FUSED && cfg->identity != ID_STANDARD) {
+    pl.identity = (cfg && (cfg->identity == ID_STANDARD || cfg->identity == ID_FOUR)) ? cfg->identity : ID_FUSED;
+    if (cfg && cfg->identity != 0 && cfg->identity != ID_FUSED && cfg->identity != ID_STANDARD &&
+        cfg->identity != ID_FOUR) {
         log_error(op, "invalid identity %d", cfg->identity);
+        return false;
+    }
+    if (pl.identity == ID_FOUR && (pl.format != FMT_F32 || (cfg->variant != 0 && cfg->variant != 3))) {
+        log_error(op, "the four-product form runs on FP32 products in the LDS-free kernel only");
         return false;
     }
     const long long m = p.m, n = p.n, k = p.k;
@@ -208,12 +213,33 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         return false;
     }
     pl.path = pl.parts > 1 ? 2 : 1;
-    if (cfg && (cfg->variant < 0 || cfg->variant > 3 ||
-                (cfg->variant == 3 && (pl.format != FMT_F32 || pl.bm != 128)))) {
+    if (cfg && (cfg->variant < 0 || cfg->variant > 4 ||
+                (cfg->variant == 3 && (pl.format != FMT_F32 || (pl.bm != 128 && pl.identity != ID_FOUR))) ||
+                (cfg->variant == 4 && pl.format != FMT_BF16))) {
         log_error(op, "invalid variant %d for this precision and tile", cfg->variant);
         return false;
     }
     pl.variant = (cfg && cfg->variant != 0) ? cfg->variant - 1 : default_variant(pl.format, pl.bm);
+    // Separate products (BF16 formats): 256 x 256 tiles of each real product over the whole k
+    // range, then merge_kernel combines the three FP32 product planes. Chosen by default when
+    // nothing is pinned, the fused plan has no K split and the 3 x (256 x 256 tiles) fill at
+    // least half the CUs (8192^3 on MI355X, complex TFLOP/s fused vs separate: bf16x3 395 vs
+    // 577, bf16x6 232 vs 319; 2048^3: 291 vs 323, 173 vs 198).
+    const bool pinned = cfg && (cfg->tile != 0 || cfg->kSplit != 0 || cfg->variant != 0);
+    const long long tiles256 = ((m + 255) / 256) * ((n + 255) / 256);
+    const bool products = (cfg && cfg->variant == 4) ||
+                          (pl.format == FMT_BF16 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2);
+    if (products) {
+        if (cfg && (cfg->kSplit > 1 || (cfg->tile != 0 && cfg->tile != 128))) {
+            log_error(op, "the separate-products form has its own tile and no K split");
+            return false;
+        }
+        pl.path = 4;
+        pl.bm = pl.bn = 256;
+        pl.parts = 1;
+        pl.mPad = (int)round_up((size_t)m, 256);
+        pl.nPad = (int)round_up((size_t)n, 256);
+    }
     if (cfg && (cfg->correction < -1 || cfg->correction > 1)) {
         log_error(op, "invalid correction flag %d", cfg->correction);
         return false;
@@ -229,14 +255,17 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     if (!mul_ok((size_t)pl.mPad, (size_t)pl.kPad, t1) || !mul_ok(t1, elem, pl.aPlaneBytes)) return false;
     if (!mul_ok((size_t)pl.nPad, (size_t)pl.kPad, t2) || !mul_ok(t2, elem, pl.bPlaneBytes)) return false;
     pl.planesA = pl.planesB = 3 * pl.pieces;
+    if (pl.identity == ID_FOUR) pl.planesB = 2;  // (Br, Bi)
     size_t total = 0, bytes = 0;
     if (!mul_ok(pl.aPlaneBytes, (size_t)pl.planesA, bytes) || !add_region(bytes, total, pl.offA)) return false;
     if (!mul_ok(pl.bPlaneBytes, (size_t)pl.planesB, bytes) || !add_region(bytes, total, pl.offB)) return false;
-    if (pl.parts > 1) {
+    if (pl.parts > 1 || pl.path == 4) {
         size_t mn;
         if (!mul_ok((size_t)pl.mPad, (size_t)pl.nPad, mn)) return false;
-        pl.partStride = 2 * mn;
-        if (!mul_ok(pl.partStride * 4, (size_t)pl.parts, bytes) || !add_region(bytes, total, pl.offPartial))
+        // K split: re and im planes per partition; separate products: one plane per product
+        pl.partStride = pl.path == 4 ? mn : 2 * mn;
+        if (!mul_ok(pl.partStride * 4, (size_t)(pl.path == 4 ? 3 : pl.parts), bytes) ||
+            !add_region(bytes, total, pl.offPartial))
             return false;
     }
     pl.kTiles = (pl.kPad + 63) / 64;
@@ -278,6 +307,7 @@ static PackArgs pack_args(const Problem& p, const Plan& pl, char* ws, int role)
     a.identity = pl.identity;
     a.format = pl.format;
     a.pieces = pl.pieces;
+    a.nplanes = isA ? pl.planesA / pl.pieces : pl.planesB / pl.pieces;
     a.planes = ws + (isA ? pl.offA : pl.offB);
     a.planeBytes = isA ? pl.aPlaneBytes : pl.bPlaneBytes;
     a.energyPartial = pl.correction ? reinterpret_cast<double*>(ws + (isA ? pl.offEPA : pl.offEPB)) : nullptr;
@@ -355,7 +385,11 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.betaZero = is_zero(p.beta) ? 1 : 0;
         g.C = p.C;
         g.ldc = p.ldc;
-        g.partial = pl.parts > 1 ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr;
+        g.partial = (pl.parts > 1 || pl.path == 4) ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr;
+        if (pl.path == 4) {
+            g.tilesM = pl.mPad / 256;
+            g.tilesN = pl.nPad / 256;
+        }
         g.partStride = pl.partStride;
         g.scaleA = pa.scaleExp;
         g.scaleB = pb.scaleExp;
@@ -369,18 +403,22 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.shadow = pl.correction && !is_zero(p.beta) ? ws + pl.offShadow : nullptr;
         if (e == hipSuccess) {
             StageTimer t(h, ST_GEMM);
-            if (pl.format == FMT_F32)
+            if (pl.path == 4)
+                e = launch_gemm_bf16_products(g, pl.pieces, s);
+            else if (pl.format == FMT_F32)
                 e = launch_gemm_f32(g, pl.bm, pl.bn, pl.variant, s);
             else if (pl.format == FMT_BF16)
                 e = launch_gemm_bf16(g, pl.bm, pl.bn, pl.pieces, pl.variant, s);
             else
                 e = launch_gemm_f16(g, pl.bm, pl.bn, s);
         }
-        if (pl.parts > 1 && e == hipSuccess) {
+        if ((pl.parts > 1 || pl.path == 4) && e == hipSuccess) {
             MergeArgs mg{};
             mg.partial = g.partial;
             mg.partStride = pl.partStride;
-            mg.parts = pl.parts;
+            mg.parts = pl.path == 4 ? 3 : pl.parts;
+            mg.products = pl.path == 4 ? 1 : 0;
+            mg.identity = pl.identity;
             mg.ldp = pl.mPad;
             mg.m = p.m;
             mg.n = p.n;
@@ -448,10 +486,10 @@ static void record_plan(aclblasHandle_t h, const Plan& pl)
     h->lastPlan[7] = (long long)pl.total;
 }
 
-static aclblasStatus_t run(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
-                           int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
-                           const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
-                           int ldc, const aclblasCgemm3mConfig* cfg, const char* op)
+aclblasStatus_t run_problem(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
+                            int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                            const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
+                            int ldc, const aclblasCgemm3mConfig* cfg, const char* op)
 {
     aclblasStatus_t st = validate(handle, ta, tb, m, n, k, alpha, lda, ldb, beta, ldc, op);
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
@@ -493,7 +531,43 @@ extern "C" aclblasStatus_t aclblasCgemm3m(aclblasHandle_t handle, aclblasOperati
                                           const aclblasComplex* B, int ldb, const aclblasComplex* beta,
                                           aclblasComplex* C, int ldc)
 {
-    return run(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, kOp);
+    return run_problem(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, kOp);
+}
+
+// C <- alpha*op(A)*op(B) + beta*C with four real products (re = Ar*Br - Ai*Bi,
+// im = Ar*Bi + Ai*Br), accumulated directly into (re, im) by the LDS-free FP32 kernel.
+// Reference: ops-blas/blas/gemm/arch35/cgemm_host.cpp:732-738 over LaunchComplexGemm
+// (:701-730). Differences from aclblasCgemm3m kept from there: a bad op returns
+// INVALID_ENUM (no remap, :713), and A and B must be non-null whenever k > 0, alpha == 0
+// included (skipProduct applies to the 3m entry only, :721-722).
+extern "C" aclblasStatus_t aclblasCgemm(aclblasHandle_t handle, aclblasOperation_t transa,
+                                        aclblasOperation_t transb, int m, int n, int k, const aclblasComplex* alpha,
+                                        const aclblasComplex* A, int lda, const aclblasComplex* B, int ldb,
+                                        const aclblasComplex* beta, aclblasComplex* C, int ldc)
+{
+    const char* op = "aclblasCgemm";
+    if (handle == nullptr) {
+        log_error(op, "handle is nullptr");
+        return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
+    }
+    const auto valid = [](aclblasOperation_t t) {
+        return t == ACLBLAS_OP_N || t == ACLBLAS_OP_T || t == ACLBLAS_OP_C;
+    };
+    if (!valid(transa) || !valid(transb)) {
+        log_error(op, "invalid transa/transb: %d/%d", (int)transa, (int)transb);
+        return ACLBLAS_STATUS_INVALID_ENUM;
+    }
+    aclblasStatus_t st = validate(handle, transa, transb, m, n, k, alpha, lda, ldb, beta, ldc, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    if (m == 0 || n == 0) {
+        Plan none;
+        record_plan(handle, none);
+        return ACLBLAS_STATUS_SUCCESS;
+    }
+    st = validate_pointers(k, A, B, C, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    const aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 0, 0, ID_FOUR, 0};
+    return run_problem(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, &cfg, op);
 }
 
 extern "C" aclblasStatus_t aclblasCgemm3mEx(aclblasHandle_t handle, aclblasOperation_t transa,
@@ -502,7 +576,8 @@ extern "C" aclblasStatus_t aclblasCgemm3mEx(aclblasHandle_t handle, aclblasOpera
                                             const aclblasComplex* B, int ldb, const aclblasComplex* beta,
                                             aclblasComplex* C, int ldc, const aclblasCgemm3mConfig* config)
 {
-    return run(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, config, "aclblasCgemm3mEx");
+    return run_problem(handle, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, config,
+                       "aclblasCgemm3mEx");
 }
 
 extern "C" aclblasStatus_t aclblasCgemm3mGetWorkspaceSize(aclblasHandle_t handle, aclblasOperation_t transa,

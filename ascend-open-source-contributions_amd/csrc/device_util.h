@@ -94,10 +94,20 @@ __device__ __forceinline__ void flag_output(unsigned long long* bitmap, long lon
 // PARTIAL: write this K partition's (re, im) planes (ld = mPad, padded extent, no checks).
 // Otherwise: alpha/beta in registers, C read only when beta != 0, rows >= m and columns
 // >= n never touched, and the rounding-correction screen on every written output.
-template <int TN, int TM, bool PARTIAL>
-__device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[3][TN][TM], int part, int iBase,
+template <int TN, int TM, bool PARTIAL, int NACC = 3>
+__device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC][TN][TM], int part, int iBase,
                                            int jBase)
 {
+    // NACC == 3: the three products, combined by the 3M identity; NACC == 2: (re, im) as
+    // accumulated by the four-product form.
+    auto combine = [&](int a, int b, int r, float& re, float& im) {
+        if constexpr (NACC == 2) {
+            re = acc[0][a][b][r];
+            im = acc[NACC - 1][a][b][r];
+        } else {
+            combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[NACC - 1][a][b][r], re, im);
+        }
+    };
     if (PARTIAL) {
         float* pr = g.partial + (size_t)part * g.partStride;
         float* pi = pr + (size_t)g.mPad * g.nPad;
@@ -110,7 +120,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[3][T
                     const int i = iBase + b * 32;
                     const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
                     float re, im;
-                    combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[2][a][b][r], re, im);
+                    combine(a, b, r, re, im);
                     const size_t o = (size_t)j * g.mPad + i;
                     pr[o] = re;
                     pi[o] = im;
@@ -128,7 +138,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[3][T
                     const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
                     if (i < g.m && j < g.n) {
                         float re, im;
-                        combine3(g.identity, acc[0][a][b][r], acc[1][a][b][r], acc[2][a][b][r], re, im);
+                        combine(a, b, r, re, im);
                         const size_t o = (size_t)j * g.ldc + i;
                         float2 c0 = make_float2(0.0f, 0.0f);
                         if (!bz) c0 = C[o];

@@ -21,15 +21,22 @@
 namespace c3m {
 
 // One output (i, j) of a K-split product: partitions summed in index order, then alpha/beta.
+// With a.products set, a.partial holds the three real product planes instead (a.partStride
+// floats apart), combined exactly as the fused tile combines its accumulators.
 __device__ __forceinline__ void merge_one(const MergeArgs& a, int i, int j)
 {
     const size_t o = (size_t)j * a.ldp + i;
     const size_t plane = a.partStride;  // floats per partition: re plane then im plane
     const size_t imOff = plane / 2;
     float re = 0.0f, im = 0.0f;
-    for (int p = 0; p < a.parts; ++p) {
-        re += a.partial[(size_t)p * plane + o];
-        im += a.partial[(size_t)p * plane + imOff + o];
+    if (a.products) {
+        // separate-products path: the combine of the fused tile, on the three product planes
+        combine3(a.identity, a.partial[o], a.partial[plane + o], a.partial[2 * plane + o], re, im);
+    } else {
+        for (int p = 0; p < a.parts; ++p) {
+            re += a.partial[(size_t)p * plane + o];
+            im += a.partial[(size_t)p * plane + imOff + o];
+        }
     }
     float2* C = reinterpret_cast<float2*>(a.C);
     const size_t co = (size_t)j * a.ldc + i;
@@ -88,6 +95,9 @@ __global__ __launch_bounds__(256) void scale_c_kernel(ScaleCArgs a)
         if (a.betaZero) {
             C[o] = make_float2(0.0f, 0.0f);
         } else {
+            // Rounded like the reference-BLAS complex product (no fused multiply-add), so
+            // beta*C is bit-exact against it, as the reference's alpha == 0 checks require.
+#pragma clang fp contract(off)
             const float2 v = C[o];
             float2 r;
             r.x = a.betaR * v.x - a.betaI * v.y;

@@ -175,17 +175,23 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_f32_kernel(GemmArgs g)
 // MFMAs of stage s, so their latency hides behind a whole stage of MFMAs. Fragments that
 // several waves of the workgroup share are read by each of them (served by the CU's vector
 // cache and L2). Same MFMA order per accumulator as f32_tile, so the results are identical.
-template <int BM, int BN, int WM, int WN>
+//
+// FOUR = true is the four-product form (ID_FOUR): A planes (Ar, Ai, -Ai), B planes (Br, Bi),
+// two accumulators per 32x32 tile taking re += Br*Ar then Bi*(-Ai), im += Bi*Ar then Br*Ai
+// per k slot; the epilogue stores them as (re, im).
+template <int BM, int BN, int WM, int WN, int NB>
 struct DirectSet {
     using T = F32Tile<BM, BN, WM, WN>;
-    f32x4 a[T::KF][3][T::TM], b[T::KF][3][T::TN];
+    f32x4 a[T::KF][3][T::TM], b[T::KF][NB][T::TN];
 };
 
-template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2>
+template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2, bool FOUR = false>
 __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int tmI, int tnI)
 {
     using T = F32Tile<BM, BN, WM, WN>;
-    using Set = DirectSet<BM, BN, WM, WN>;
+    constexpr int NB = FOUR ? 2 : 3;    // B planes
+    constexpr int NACC = FOUR ? 2 : 3;  // accumulators per 32x32 output tile
+    using Set = DirectSet<BM, BN, WM, WN, NB>;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave % WM, wn = wave / WM;
@@ -210,15 +216,17 @@ __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int
 #pragma unroll
                 for (int i = 0; i < T::TM; ++i)
                     S.a[kfl][q][i] = *reinterpret_cast<const f32x4*>(a + i * 1024 + laneB);
+                if (q < NB) {
 #pragma unroll
-                for (int j = 0; j < T::TN; ++j)
-                    S.b[kfl][q][j] = *reinterpret_cast<const f32x4*>(b + j * 1024 + laneB);
+                    for (int j = 0; j < T::TN; ++j)
+                        S.b[kfl][q < NB ? q : 0][j] = *reinterpret_cast<const f32x4*>(b + j * 1024 + laneB);
+                }
             }
     };
 
-    f32x16 acc[3][T::TN][T::TM];
+    f32x16 acc[NACC][T::TN][T::TM];
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < NACC; ++q)
 #pragma unroll
         for (int a = 0; a < T::TN; ++a)
 #pragma unroll
@@ -226,46 +234,67 @@ __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[q][a][b][r] = 0.0f;
 
+    // term t: accumulator, B plane, A plane
+    constexpr int TACC[4] = {0, 1, 0, 1}, TBP[4] = {0, 1, 1, 0}, TAP[4] = {0, 0, 2, 1};
     auto mfma = [&](const Set& S) {
 #pragma unroll
         for (int kfl = 0; kfl < T::KF; ++kfl)
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
+            for (int s = 0; s < 4; ++s) {
+                if constexpr (FOUR) {
 #pragma unroll
-                for (int q = 0; q < 3; ++q)
+                    for (int t = 0; t < 4; ++t)
 #pragma unroll
-                    for (int a = 0; a < T::TN; ++a)
+                        for (int a = 0; a < T::TN; ++a)
 #pragma unroll
-                        for (int b = 0; b < T::TM; ++b)
-                            acc[q][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(S.b[kfl][q][a][s], S.a[kfl][q][b][s],
-                                                                                acc[q][a][b], 0, 0, 0);
+                            for (int b = 0; b < T::TM; ++b)
+                                acc[TACC[t]][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                                    S.b[kfl][TBP[t]][a][s], S.a[kfl][TAP[t]][b][s], acc[TACC[t]][a][b], 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+#pragma unroll
+                        for (int a = 0; a < T::TN; ++a)
+#pragma unroll
+                            for (int b = 0; b < T::TM; ++b)
+                                acc[q][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                                    S.b[kfl][q < NB ? q : 0][a][s], S.a[kfl][q][b][s], acc[q][a][b], 0, 0, 0);
+                }
+            }
     };
 
     // Stages in pairs (set 0, set 1) with one exit, so the accumulators keep their registers;
     // an odd last stage is already loaded into set 0.
+    // Loads are unconditional (the last pair's look-ahead is clamped to the last stage): a
+    // load skipped on one path would make the wait before the next MFMAs cover the loads just
+    // issued. Scheduling barriers keep each stage's loads ahead of the MFMAs.
     Set s0, s1;
     const int nst = ks1 - ks0;
     if (nst > 0) load(s0, ks0);
     int st = 0;
     for (; st + 1 < nst; st += 2) {
         load(s1, ks0 + st + 1);
+        __builtin_amdgcn_sched_barrier(0);
         mfma(s0);
-        if (st + 2 < nst) load(s0, ks0 + st + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        load(s0, ks0 + min(st + 2, nst - 1));
+        __builtin_amdgcn_sched_barrier(0);
         mfma(s1);
+        __builtin_amdgcn_sched_barrier(0);
     }
     if (st < nst) mfma(s0);
 
     const int iBase = tmI * BM + wm * (BM / WM) + (lane & 31);
     const int jBase = tnI * BN + wn * (BN / WN) + 4 * (lane >> 5);
-    store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
+    store_tile<T::TN, T::TM, PARTIAL, NACC>(g, acc, part, iBase, jBase);
 }
 
-template <int BM, int BN, bool PARTIAL, int WM, int WN, int OCC>
+template <int BM, int BN, bool PARTIAL, int WM, int WN, int OCC, bool FOUR = false>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm3m_f32_direct_kernel(GemmArgs g)
 {
     int part, tmI, tnI;
     map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, part, tmI, tnI);
-    f32_tile_direct<BM, BN, PARTIAL, WM, WN>(g, part, tmI, tnI);
+    f32_tile_direct<BM, BN, PARTIAL, WM, WN, FOUR>(g, part, tmI, tnI);
 }
 
 // Grouped batch: a persistent grid pulls tiles of many independent problems from one
@@ -347,23 +376,32 @@ static hipError_t launch_one(const GemmArgs& g, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int BM, int BN, bool PARTIAL, int WM, int WN, int OCC>
+template <int BM, int BN, bool PARTIAL, int WM, int WN, int OCC, bool FOUR = false>
 static hipError_t launch_direct(const GemmArgs& g, hipStream_t s)
 {
     const int blocks = g.tilesM * g.tilesN * g.parts;
-    hipLaunchKernelGGL((gemm3m_f32_direct_kernel<BM, BN, PARTIAL, WM, WN, OCC>), dim3(blocks), dim3(64 * WM * WN), 0,
-                       s, g);
+    hipLaunchKernelGGL((gemm3m_f32_direct_kernel<BM, BN, PARTIAL, WM, WN, OCC, FOUR>), dim3(blocks),
+                       dim3(64 * WM * WN), 0, s, g);
     return hipGetLastError();
 }
 
 // 128x128 tiles: variant 0: 4 waves through LDS (64x64 per wave); 1: 8 waves through LDS
 // (64x32 per wave, two per SIMD); 2: LDS-free, 4 waves (the default). Measured at 8192^3 on
-// MI355X, products kernel alone, complex TFLOP/s: 151.7, 162.5, 177.2. The LDS-free form
+// MI355X, products kernel alone, complex TFLOP/s: 151.7, 162.5, 185.6. The LDS-free form
 // with 8 waves (92) or 64x64 tiles (85, against 150 through LDS) loses: its wave tiles are
 // too small for the operands it re-reads from the caches, so it is not built.
 hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s)
 {
     const bool partial = g.parts > 1;
+    if (g.identity == ID_FOUR) {  // four products: LDS-free kernels only
+        if (bm == 128 && bn == 128)
+            return partial ? launch_direct<128, 128, true, 2, 2, 1, true>(g, s)
+                           : launch_direct<128, 128, false, 2, 2, 1, true>(g, s);
+        if (bm == 64 && bn == 64)
+            return partial ? launch_direct<64, 64, true, 2, 2, 2, true>(g, s)
+                           : launch_direct<64, 64, false, 2, 2, 2, true>(g, s);
+        return hipErrorInvalidValue;
+    }
     if (bm == 128 && bn == 128 && variant == 2)
         return partial ? launch_direct<128, 128, true, 2, 2, 1>(g, s) : launch_direct<128, 128, false, 2, 2, 1>(g, s);
     if (bm == 128 && bn == 128) {

@@ -223,6 +223,118 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
     store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
 }
 
+// Separate-products form for the BF16 formats. One workgroup computes one 256 x 256 tile of
+// ONE real product P_q; its 2 x 2 waves each own 128 x 128 outputs (16 accumulators of 32 x 32,
+// all 256 AGPRs). Holding one product per wave instead of three doubles the MFMAs each loaded
+// fragment feeds (4 per term instead of 2), so the operand bytes per MFMA halve and the waves
+// can load their fragments straight from the packed planes into registers: no LDS, no
+// barriers, NSET register sets so the loads of stage s + NSET - 1 are in flight during the
+// MFMAs of stage s. Per accumulator the MFMA order is the fused kernel's (stage by stage, the
+// same term order), so each P_q is bit-identical to the fused kernel's product; the FP32
+// planes P_q (ld = mPad) are combined and alpha/beta applied by merge_kernel.
+template <int NP, int NSET>
+__global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_kernel(GemmArgs g)
+{
+    constexpr int TM = 4, TN = 4;
+    constexpr int NTERM = NP == 2 ? 3 : 6;
+    constexpr int TB[6] = {1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
+    constexpr int TA[6] = {NP == 2 ? 0 : 1, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
+    int q, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, q, tmI, tnI);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const size_t aRowB = (size_t)(g.mPad / 32) * 1024;  // bytes per 16-deep k fragment row
+    const size_t bRowB = (size_t)(g.nPad / 32) * 1024;
+    const char* aW = reinterpret_cast<const char*>(g.Ap) + (size_t)(q * NP) * g.aPlaneBytes +
+                     (size_t)(tmI * 8 + wm * TM) * 1024;
+    const char* bW = reinterpret_cast<const char*>(g.Bp) + (size_t)(q * NP) * g.bPlaneBytes +
+                     (size_t)(tnI * 8 + wn * TN) * 1024;
+    const unsigned laneB = (unsigned)lane * 16u;
+
+    struct Set {
+        u32x4 a[NP][TM], b[NP][TN];
+    };
+    auto load = [&](Set& S, int ks) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const char* a = aW + (size_t)p * g.aPlaneBytes + (size_t)ks * aRowB;
+            const char* b = bW + (size_t)p * g.bPlaneBytes + (size_t)ks * bRowB;
+#pragma unroll
+            for (int i = 0; i < TM; ++i) S.a[p][i] = *reinterpret_cast<const u32x4*>(a + i * 1024 + laneB);
+#pragma unroll
+            for (int j = 0; j < TN; ++j) S.b[p][j] = *reinterpret_cast<const u32x4*>(b + j * 1024 + laneB);
+        }
+    };
+    f32x16 acc[TN][TM];
+#pragma unroll
+    for (int a = 0; a < TN; ++a)
+#pragma unroll
+        for (int b = 0; b < TM; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+    auto mfma = [&](const Set& S) {
+#pragma unroll
+        for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+            for (int a = 0; a < TN; ++a)
+#pragma unroll
+                for (int b = 0; b < TM; ++b) acc[a][b] = mfma16(S.b[TB[t]][a], S.a[TA[t]][b], acc[a][b], false);
+    };
+
+    // Ring of NSET register sets: sets 0..NSET-2 preloaded; stage st computes from set
+    // st % NSET after issuing the loads of stage st + NSET - 1 into the set freed last.
+    const int nst = g.kSteps;
+    Set S[NSET];
+#pragma unroll
+    for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nst - 1));
+    int st = 0;
+    for (; st + NSET <= nst; st += NSET) {
+#pragma unroll
+        for (int u = 0; u < NSET; ++u) {
+            // Unconditional (clamped to the last stage): a load skipped on one path would make
+            // the wait before the next MFMAs cover the loads just issued.
+            load(S[(u + NSET - 1) % NSET], min(st + u + NSET - 1, nst - 1));
+            // Keep the loads ahead of the MFMAs: the scheduler would otherwise sink each load
+            // next to its first use and wait on it there.
+            __builtin_amdgcn_sched_barrier(0);
+            mfma(S[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u + 1 < NSET; ++u)
+        if (st + u < nst) mfma(S[u]);
+
+    float* P = g.partial + (size_t)q * g.partStride;
+    const int iBase = tmI * 256 + wm * 128 + (lane & 31);
+    const int jBase = tnI * 256 + wn * 128 + 4 * (lane >> 5);
+#pragma unroll
+    for (int a = 0; a < TN; ++a)
+#pragma unroll
+        for (int b = 0; b < TM; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = iBase + b * 32;
+                const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+                P[(size_t)j * g.mPad + i] = acc[a][b][r];
+            }
+}
+
+// g.tilesM / g.tilesN count 256 x 256 tiles (mPad, nPad multiples of 256); g.partial holds
+// three FP32 planes of mPad x nPad, g.partStride floats apart; the whole k range (no split).
+hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, hipStream_t s)
+{
+    const int blocks = 3 * g.tilesM * g.tilesN;
+    if (pieces == 2)
+        hipLaunchKernelGGL((gemm3m_lowp_products_kernel<2, 3>), dim3(blocks), dim3(256), 0, s, g);
+    else if (pieces == 3)
+        hipLaunchKernelGGL((gemm3m_lowp_products_kernel<3, 2>), dim3(blocks), dim3(256), 0, s, g);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 template <int BM, int BN, int NP, bool F16S, bool PARTIAL, int WM = 2, int WN = 2>
 static hipError_t launch_one(const GemmArgs& g, hipStream_t s)
 {

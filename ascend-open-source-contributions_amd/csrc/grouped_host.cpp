@@ -52,17 +52,38 @@ static uint64_t fnv1a(const void* data, size_t bytes)
 // pinned host buffer on the stream; later calls with the same content reuse the device copy
 // without any copy, so a hipGraph captured after one eager call holds kernels only. If the
 // first call is itself being captured, the copy becomes a graph node and a later eager call
-// uploads again.
+// uploads again. A block used while capturing is marked and never evicted; otherwise, when
+// more than kMaxStaged blocks are held, the least recently used one is freed after the
+// device has finished the work queued so far (eviction never happens during a capture).
 static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>& blob, hipStream_t s,
                                      hipError_t* err)
 {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const bool capturing = hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
     const uint64_t key = fnv1a(blob.data(), blob.size());
-    _aclblas_handle::Staged* hit = nullptr;
-    for (auto& st : h->staged)
-        if (st.hash == key && st.bytes == blob.size() && std::memcmp(st.host, blob.data(), blob.size()) == 0)
-            hit = &st;
-    if (hit == nullptr) {
-        _aclblas_handle::Staged st{key, blob.size(), nullptr, nullptr, false};
+    size_t idx = h->staged.size();
+    for (size_t i = 0; i < h->staged.size(); ++i) {
+        const auto& st = h->staged[i];
+        if (st.hash == key && st.bytes == blob.size() && std::memcmp(st.host, blob.data(), blob.size()) == 0) idx = i;
+    }
+    if (idx == h->staged.size()) {
+        if (!capturing) {
+            size_t loose = 0;
+            for (const auto& st : h->staged) loose += st.captured ? 0 : 1;
+            if (loose >= kMaxStaged) {
+                size_t victim = h->staged.size();
+                for (size_t i = 0; i < h->staged.size(); ++i)
+                    if (!h->staged[i].captured &&
+                        (victim == h->staged.size() || h->staged[i].lastUse < h->staged[victim].lastUse))
+                        victim = i;
+                *err = hipDeviceSynchronize();
+                if (*err != hipSuccess) return nullptr;
+                (void)hipHostFree(h->staged[victim].host);
+                (void)hipFree(h->staged[victim].device);
+                h->staged.erase(h->staged.begin() + (std::ptrdiff_t)victim);
+            }
+        }
+        _aclblas_handle::Staged st{key, blob.size(), nullptr, nullptr, false, false, 0};
         if (hipHostMalloc(&st.host, blob.size(), hipHostMallocDefault) != hipSuccess) {
             *err = hipGetLastError();
             return nullptr;
@@ -74,15 +95,17 @@ static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>&
         }
         std::memcpy(st.host, blob.data(), blob.size());
         h->staged.push_back(st);
-        hit = &h->staged.back();
+        idx = h->staged.size() - 1;
     }
-    if (!hit->uploaded) {
-        *err = hipMemcpyAsync(hit->device, hit->host, hit->bytes, hipMemcpyHostToDevice, s);
+    _aclblas_handle::Staged& hit = h->staged[idx];
+    hit.lastUse = ++h->stageClock;
+    if (capturing) hit.captured = true;
+    if (!hit.uploaded) {
+        *err = hipMemcpyAsync(hit.device, hit.host, hit.bytes, hipMemcpyHostToDevice, s);
         if (*err != hipSuccess) return nullptr;
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) hit->uploaded = true;
+        if (!capturing) hit.uploaded = true;
     }
-    return static_cast<const char*>(hit->device);
+    return static_cast<const char*>(hit.device);
 }
 
 struct GroupedLayout {
@@ -97,6 +120,29 @@ struct GroupedLayout {
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Plan of one grouped member: the tuned table's (tile, K partitions) for grouped members,
+// else 128 tiles for members of at least 256 x 256 and 64 tiles below, no K split; no
+// rounding correction; the caller's tile/kSplit/identity override when given. The one-problem
+// form of the same plan (variant: LDS-free kernel for 128 tiles, as the grouped kernel
+// uses) gives byte-identical results, which the strided-batched entry relies on.
+aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over)
+{
+    aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0, 1};
+    int tt = 0, tp = 0;
+    if (over != nullptr && over->tile != 0) {
+        cfg.tile = over->tile;
+        cfg.kSplit = over->kSplit != 0 ? over->kSplit : 1;
+    } else if (tuned_lookup(m, n, k, true, &tt, &tp)) {
+        cfg.tile = tt;
+        cfg.kSplit = tp;
+    } else {
+        cfg.tile = (m >= 256 && n >= 256) ? 128 : 64;
+    }
+    if (over != nullptr && over->identity != 0) cfg.identity = over->identity;
+    cfg.variant = cfg.tile == 128 ? 3 : 1;
+    return cfg;
+}
 
 static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasOperation_t* ta,
                                     const aclblasOperation_t* tb, const int* m, const int* n, const int* k,
@@ -124,18 +170,7 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             if (!(p.beta.real == 1.0f && p.beta.imag == 0.0f)) L.scaleOnly.push_back(p);
             continue;
         }
-        aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0, 1};
-        int tt = 0, tp = 0;
-        if (over != nullptr && over->tile != 0) {
-            cfg.tile = over->tile;
-            cfg.kSplit = over->kSplit != 0 ? over->kSplit : 1;
-        } else if (tuned_lookup(p.m, p.n, p.k, true, &tt, &tp)) {
-            cfg.tile = tt;
-            cfg.kSplit = tp;
-        } else {
-            cfg.tile = (p.m >= 256 && p.n >= 256) ? 128 : 64;
-        }
-        if (over != nullptr && over->identity != 0) cfg.identity = over->identity;
+        const aclblasCgemm3mConfig cfg = grouped_member_config(p.m, p.n, p.k, over);
         Plan pl;
         if (!make_plan(p, &cfg, h->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
         L.probs.push_back(p);
@@ -193,7 +228,7 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasH
     return ACLBLAS_STATUS_SUCCESS;
 }
 
-static aclblasStatus_t grouped_run(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
+aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
                                    const aclblasOperation_t* transb, const int* m, const int* n, const int* k,
                                    const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
                                    const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
@@ -269,6 +304,7 @@ static aclblasStatus_t grouped_run(aclblasHandle_t handle, int count, const aclb
                 a.identity = pl.identity;
                 a.format = pl.format;
                 a.pieces = pl.pieces;
+                a.nplanes = 3;  // grouped members use a 3M identity
                 a.planes = pws + (isA ? pl.offA : pl.offB);
                 a.planeBytes = isA ? pl.aPlaneBytes : pl.bPlaneBytes;
                 a.energyPartial = nullptr;
@@ -409,8 +445,10 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle
 {
     const char* op = "aclblasCgemm3mGroupedBatchedEx";
     if (config != nullptr && ((config->tile != 0 && config->tile != 64 && config->tile != 128) ||
-                              config->kSplit < 0 || config->kSplit > 16)) {
-        log_error(op, "invalid grouped config (tile %d, kSplit %d)", config->tile, config->kSplit);
+                              config->kSplit < 0 || config->kSplit > 16 || config->identity < 0 ||
+                              config->identity > ID_STANDARD)) {
+        log_error(op, "invalid grouped config (tile %d, kSplit %d, identity %d)", config->tile, config->kSplit,
+                  config->identity);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     return grouped_run(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,

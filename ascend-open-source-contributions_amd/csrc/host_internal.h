@@ -28,15 +28,20 @@ struct _aclblas_handle {
     long long timedCalls = 0;
     // Grouped-batch descriptor blocks, keyed by content: a pinned host copy and a device
     // copy that is uploaded once. Later calls with the same problems (and hipGraph captures
-    // of them) only read the device copy. Freed by aclblasDestroy.
+    // of them) only read the device copy. At most kMaxStaged blocks that no capture used are
+    // kept (least recently used evicted); blocks a capture used live until aclblasDestroy,
+    // since a graph may replay them at any time.
     struct Staged {
         uint64_t hash;
         size_t bytes;
         void* host;
         void* device;
         bool uploaded;
+        bool captured;
+        uint64_t lastUse;
     };
     std::vector<Staged> staged;
+    uint64_t stageClock = 0;
 };
 
 namespace c3m {
@@ -44,6 +49,7 @@ namespace c3m {
 constexpr size_t kDefaultWorkspace = 32u << 20;  // cann_ops_blas.h:34-36
 constexpr size_t kAlign = 256;
 constexpr int kVersion = 10000;
+constexpr size_t kMaxStaged = 32;  // descriptor blocks kept per handle besides captured ones
 
 void log_error(const char* op, const char* fmt, ...);
 
@@ -63,7 +69,7 @@ class StageTimer {
 
 // A full execution plan for one product (pure function of the arguments and CU count).
 struct Plan {
-    int path = 0;         // 0 nothing, 1 fused tile, 2 K-split, 3 beta scale
+    int path = 0;         // 0 nothing, 1 fused tile, 2 K-split, 3 beta scale, 4 separate products
     int precision = 0;    // aclblasCgemm3mPrecision_t
     int format = FMT_F32;
     int pieces = 1;
@@ -106,5 +112,20 @@ bool is_zero(const aclblasComplex& z);
 
 // Autotuned plan table (tuned_table.h): tile and K partitions for a (log2 m*n, log2 k) bin.
 bool tuned_lookup(long long m, long long n, long long k, bool grouped, int* tile, int* parts);
+
+// One aclblasCgemm3m(Ex) call: validation, plan, workspace, launches (cgemm3m_host.cpp).
+aclblasStatus_t run_problem(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
+                            int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                            const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
+                            int ldc, const aclblasCgemm3mConfig* cfg, const char* op);
+// One grouped batch of FP32 problems (grouped_host.cpp); over: optional plan override.
+aclblasStatus_t grouped_run(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
+                            const aclblasOperation_t* transb, const int* m, const int* n, const int* k,
+                            const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
+                            const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
+                            aclblasComplex* const* C, const int* ldc, int precision,
+                            const aclblasCgemm3mConfig* over, const char* op);
+// The plan a grouped member of this shape gets, as a one-problem config.
+aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over);
 
 }  // namespace c3m

@@ -29,7 +29,9 @@ enum PlaneFormat : int { FMT_F32 = 0, FMT_BF16 = 1, FMT_F16 = 2 };
 //   standard identity A: (Ar, Ai, Ar+Ai)   B: (Br, Bi, Br+Bi)      re = P0-P1, im = P2-P0-P1
 // (textbook 3M identities; the reference uses both: cgemm_deinterleave_common.h:17-51,
 //  cgemm_compute_kernel.cpp:406-428, cgemm_epilogue_kernel.cpp:255-268)
-enum Identity : int { ID_FUSED = 1, ID_STANDARD = 2 };
+// ID_FOUR: four real products, re = Ar*Br - Ai*Bi, im = Ar*Bi + Ai*Br, accumulated directly
+// into (re, im): A planes (Ar, Ai, -Ai), B planes (Br, Bi).
+enum Identity : int { ID_FUSED = 1, ID_STANDARD = 2, ID_FOUR = 3 };
 
 struct PackArgs {
     const void* src;      // interleaved complex64, column-major, leading dim ld
@@ -42,6 +44,7 @@ struct PackArgs {
     int identity;         // Identity
     int format;           // PlaneFormat
     int pieces;           // BF16: 2 (hi, lo) or 3 (hi, mid, lo); F32/F16: 1
+    int nplanes;          // planes written: 3, or 2 for the B operand of ID_FOUR
     void* planes;         // base of plane 0; plane q piece p at planes + (q*pieces+p)*planeBytes
     size_t planeBytes;
     double* energyPartial; // optional [kPad/64][mnPad]: sum over 64 k of |x|^2 per row
@@ -90,6 +93,8 @@ struct GemmArgs {
 
 struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/beta
     const float* partial; size_t partStride; int parts; int ldp;
+    int products;       // 1: partial holds the three real products P0, P1, P2 (one plane each)
+    int identity;       // Identity used to combine P0..P2 when products == 1
     int m, n;
     float alphaR, alphaI, betaR, betaI; int betaZero;
     void* C; long long ldc;

@@ -23,6 +23,7 @@ __device__ __forceinline__ float plane_value(int role, int identity, int q, floa
     if (identity == ID_STANDARD) {
         return q == 0 ? re : (q == 1 ? im : re + im);
     }
+    if (identity == ID_FOUR) return q == 0 ? re : (q == 1 ? im : -im);  // A: (Ar, Ai, -Ai); B: (Br, Bi)
     if (role == 0) return q == 0 ? re + im : (q == 1 ? re : im);  // A: (Ar+Ai, Ar, Ai)
     return q == 0 ? re : (q == 1 ? im - re : re + im);            // B: (Br, Bi-Br, Br+Bi)
 }
@@ -84,6 +85,7 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
             const size_t idx = ((size_t)kf * mnFrags + (size_t)(r0 >> 5) + mfl) * 64 + lane;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
+                if (q >= a.nplanes) break;
                 f32x4 out;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
@@ -104,6 +106,7 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
             const size_t idx = ((size_t)kf * mnFrags + (size_t)(r0 >> 5) + mfl) * 64 + lane;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
+                if (q >= a.nplanes) break;
                 float x[8];
 #pragma unroll
                 for (int s = 0; s < 8; ++s) {

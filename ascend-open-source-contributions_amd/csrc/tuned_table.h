@@ -22,7 +22,7 @@ static const TunedEntry kTunedTable[] = {
     {14, 6, 1, 64, 1},
     {14, 8, 1, 64, 1},
     {14, 10, 1, 64, 1},
-    {14, 11, 1, 128, 4},
+    {14, 11, 1, 64, 1},
     {14, 12, 0, 64, 16},
     {16, 6, 1, 128, 1},
     {16, 8, 0, 64, 1},
@@ -40,7 +40,7 @@ static const TunedEntry kTunedTable[] = {
     {20, 10, 0, 128, 4},
     {20, 11, 0, 64, 2},
     {22, 8, 0, 128, 1},
-    {22, 11, 0, 128, 2},
+    {22, 11, 0, 128, 1},
     {24, 9, 0, 128, 1},
     {24, 12, 0, 128, 1},
     {26, 13, 0, 128, 1},

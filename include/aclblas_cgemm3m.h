@@ -7,6 +7,8 @@
  *   - handle functions                          ops-blas/include/cann_ops_blas.h:32-110
  *   - aclblasCgemm3m                            ops-blas/include/cann_ops_blas.h:497-500,
  *                                               ops-blas/blas/gemm/arch35/cgemm3m_host.cpp:17-23
+ *   - aclblasCgemm (four products)              ops-blas/include/cann_ops_blas.h:492-495
+ *   - aclblasCgemm3mStridedBatched              ops-blas/include/cann_ops_blas.h:652-655
  * Everything after the "extensions" banner has no counterpart in the reference; each
  * declaration says which caller it is for.
  *
@@ -96,6 +98,30 @@ aclblasStatus_t aclblasCgemm3m(aclblasHandle_t handle, aclblasOperation_t transa
                                const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
                                int ldc);
 
+/* C <- alpha*op(A)*op(B) + beta*C with four real products (re = Ar*Br - Ai*Bi,
+ * im = Ar*Bi + Ai*Br). ops-blas/include/cann_ops_blas.h:492-495; contract
+ * ops-blas/blas/gemm/README.md (aclblasCgemm section); entry
+ * ops-blas/blas/gemm/arch35/cgemm_host.cpp:732-738. Unlike aclblasCgemm3m: a bad op returns
+ * ACLBLAS_STATUS_INVALID_ENUM, and A and B must be non-null whenever k > 0 (alpha == 0
+ * included). */
+aclblasStatus_t aclblasCgemm(aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb, int m,
+                             int n, int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                             const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
+                             int ldc);
+
+/* C_i <- alpha*op(A_i)*op(B_i) + beta*C_i, i = 0..batchCount-1, with A_i = A + i*strideA,
+ * B_i = B + i*strideB, C_i = C + i*strideC (complex elements; zero and negative strides allowed).
+ * ops-blas/include/cann_ops_blas.h:652-655; contract ops-blas/blas/gemm/README.md, section
+ * aclblasCgemm3mStridedBatched: a bad op returns ACLBLAS_STATUS_INVALID_ENUM (not remapped);
+ * m, n or batchCount == 0 returns SUCCESS after the argument checks; overlapping output
+ * batches (for example strideC == 0) are updated in batch order. */
+aclblasStatus_t aclblasCgemm3mStridedBatched(aclblasHandle_t handle, aclblasOperation_t transa,
+                                             aclblasOperation_t transb, int m, int n, int k,
+                                             const aclblasComplex* alpha, const aclblasComplex* A, int lda,
+                                             int64_t strideA, const aclblasComplex* B, int ldb, int64_t strideB,
+                                             const aclblasComplex* beta, aclblasComplex* C, int ldc,
+                                             int64_t strideC, int batchCount);
+
 /* ========================= extensions (no reference counterpart) ========================= */
 
 /* Arithmetic used for the three real products. FP32 is what aclblasCgemm3m runs. */
@@ -112,10 +138,12 @@ typedef struct {
     int tile;       /* 0 auto, 64 or 128: output tile edge */
     int kSplit;     /* 0 auto, else 1..16 K partitions reduced in fixed order */
     int correction; /* 0 auto, 1 force on, -1 force off (FP32 only) */
-    int identity;   /* 0 auto (fused identity), 1 fused, 2 standard 3M identity */
+    int identity;   /* 0 auto (fused identity), 1 fused, 2 standard 3M identity, 3 four real
+                       products (FP32 only; what aclblasCgemm uses) */
     int variant;    /* 0 auto; 128x128 tiles: 1 four waves through LDS, 2 eight waves through
                        LDS (two per SIMD), 3 four waves loading fragments straight into
-                       registers, no LDS (FP32 only) */
+                       registers, no LDS (FP32 only); 4 separate products: each real product
+                       on its own 256x256 tiles, then a combine pass (bf16x3/bf16x6 only) */
 } aclblasCgemm3mConfig;
 
 /* For callers that choose the arithmetic or pin the plan (bench, tests, autotuner, the

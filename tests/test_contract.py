@@ -114,7 +114,8 @@ def test_validation_order(h):
 
 def test_config_validation(h):
     for cfg in (cgemm3m.config(precision=9), cgemm3m.config(tile=96), cgemm3m.config(k_split=17),
-                cgemm3m.config(correction=2), cgemm3m.config(identity=5), cgemm3m.config(variant=4),
+                cgemm3m.config(correction=2), cgemm3m.config(identity=5), cgemm3m.config(variant=5), cgemm3m.config(variant=4),
+                cgemm3m.config("bf16x3", variant=4, k_split=2), cgemm3m.config("bf16x3", variant=4, tile=64),
                 cgemm3m.config("bf16x3", variant=3), cgemm3m.config(tile=64, variant=3)):
         st = cgemm3m.cgemm3m(h, "N", "N", 16, 16, 16, 1, FAKE, 16, FAKE, 16, 0, FAKE, 16, cfg, check=False)
         assert st == _lib.INVALID_VALUE
@@ -143,3 +144,65 @@ def test_header_is_plain_c99(tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert subprocess.run([str(tmp_path / "h")]).returncode == 0
+
+
+def sb_call(h, ta="N", tb="N", m=16, n=16, k=16, alpha=1, A=FAKE, lda=16, sa=256, B=FAKE, ldb=16, sb=256, beta=0,
+            C=FAKE, ldc=16, sc=256, batch=2):
+    return cgemm3m.strided_batched(h, ta, tb, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, batch,
+                                   check=False)
+
+
+@pytest.mark.parametrize("kw,expect", [
+    (dict(ta=999), 10),                                    # INVALID_ENUM here (README: not remapped)
+    (dict(tb=999, m=-1), 10),                              # enum before dimensions
+    (dict(batch=-1), 3),
+    (dict(m=-1), 3), (dict(k=-1, ldb=1), 3),
+    (dict(lda=8), 3), (dict(ldc=8), 3),
+    (dict(alpha=None), 3), (dict(beta=None), 3),
+    (dict(batch=0, A=None, B=None, C=None), 0),            # quick return after the checks
+    (dict(batch=0, alpha=None), 3),                        # ... which still cover scalar pointers
+    (dict(m=0, lda=1, ldc=1, A=None, C=None), 0),
+    (dict(n=0, B=None, C=None), 0),
+    (dict(C=None), 3),
+    (dict(A=None), 3), (dict(B=None), 3),
+])
+def test_strided_batched_status_contract(h, kw, expect):
+    """Status rows of aclblasCgemm3mStridedBatched (ops-blas/blas/gemm/README.md, its
+    constraints list); every case fails or returns before touching memory."""
+    assert sb_call(h, **kw) == expect
+
+
+def test_strided_batched_null_handle():
+    L = _lib.load()
+    one = _lib.Complex(1, 0)
+    st = L.aclblasCgemm3mStridedBatched(None, 111, 111, 1, 1, 1, ctypes.byref(one), None, 1, 0, None, 1, 0,
+                                        ctypes.byref(one), None, 1, 0, 1)
+    assert st == _lib.HANDLE_IS_NULLPTR
+
+
+def cg_call(h, ta="N", tb="N", m=16, n=16, k=16, alpha=1, A=FAKE, lda=16, B=FAKE, ldb=16, beta=0, C=FAKE, ldc=16):
+    return cgemm3m.cgemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, check=False)
+
+
+@pytest.mark.parametrize("kw,expect", [
+    (dict(ta=999), 10), (dict(tb=999), 10),                # INVALID_ENUM, not remapped (cgemm_host.cpp:713)
+    (dict(ta=999, m=-1), 10),
+    (dict(m=-1), 3), (dict(lda=8), 3), (dict(ldc=8), 3), (dict(alpha=None), 3), (dict(beta=None), 3),
+    (dict(m=0, lda=1, ldc=1, A=None, C=None), 0),
+    (dict(C=None), 3),
+    (dict(A=None), 3), (dict(B=None), 3),
+    (dict(alpha=0, A=None), 3),                            # k > 0: A required even when alpha == 0
+    (dict(k=0, A=None, B=None, ldb=1, beta=1), 0),         # k == 0, beta == 1: nothing to do
+    (dict(precision_ok=True), None),
+])
+def test_cgemm_status_contract(h, kw, expect):
+    """aclblasCgemm's contract (ops-blas/blas/gemm/README.md, aclblasCgemm; shared validation
+    cgemm_host.cpp:589-634, :687-730 with forceThreeProducts == false)."""
+    if kw.get("precision_ok"):
+        # the four-product form is FP32-only and LDS-free: other pins are rejected
+        for cfg in (cgemm3m.config("bf16x3", identity=3), cgemm3m.config(identity=3, variant=1),
+                    cgemm3m.config(identity=4)):
+            st = cgemm3m.cgemm3m(h, "N", "N", 16, 16, 16, 1, FAKE, 16, FAKE, 16, 0, FAKE, 16, cfg, check=False)
+            assert st == _lib.INVALID_VALUE
+        return
+    assert cg_call(h, **kw) == expect

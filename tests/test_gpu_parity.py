@@ -53,7 +53,13 @@ def test_op_sweep_padded(gpu, ta, tb):
 
 
 def test_square_2048_fused_with_correction(gpu):
-    out, plan = _case(gpu, "N", "N", 2048, 2048, 2048, ALPHA2, BETA2, 2048, 2048, 2048, 20260200, label="sq2048")
+    """2048^3 on the fused tile (pinned: no K split) with the correction screen on (auto at
+    k >= 1024)."""
+    import cgemm3m
+
+    cfg = cgemm3m.config(tile=128, k_split=1)
+    out, plan = _case(gpu, "N", "N", 2048, 2048, 2048, ALPHA2, BETA2, 2048, 2048, 2048, 20260200, cfg=cfg,
+                      label="sq2048")
     assert plan["path"] == 1 and plan["correction"] == 1
 
 
@@ -151,15 +157,17 @@ def test_identity_and_tile(gpu, identity, tile):
           cfg=cfg, label="id%d-t%d" % (identity, tile))
 
 
-@pytest.mark.parametrize("prec,variants", [("fp32", (1, 2, 3)), ("bf16x3", (1, 2))])
+@pytest.mark.parametrize("prec,variants", [("fp32", (1, 2, 3)), ("bf16x3", (1, 2, 4)), ("bf16x6", (2, 4))])
 @pytest.mark.parametrize("parts", [1, 3])
 def test_kernel_variants_identical(gpu, prec, variants, parts):
-    """The 128x128 product kernels (four or eight waves through LDS, four waves loading
-    fragments straight into registers) issue the same MFMAs per accumulator in the same
-    order, so their outputs are byte-identical; the first also passes the oracle check."""
+    """The product kernels (four or eight waves through LDS, four waves loading fragments
+    straight into registers, each real product on its own 256x256 tiles then combined) issue
+    the same MFMAs per accumulator in the same order and combine the products the same way,
+    so their outputs are byte-identical; the first also passes the oracle check."""
     import cgemm3m
 
     m, n, k = 1000, 700, 1100
+    variants = [v for v in variants if not (v == 4 and parts > 1)]  # no K split there
     outs = []
     for v in variants:
         cfg = cgemm3m.config(prec, tile=128, k_split=parts, variant=v)
@@ -303,3 +311,23 @@ def test_correction_fixes_flagged_and_counts(gpu):
     on, plan_on = outs[1]
     assert plan_on["flagged"] >= 1
     assert abs(complex(on[0]) - z[0]) <= 1e-6 * abs(z[0]) + 1e-30
+
+
+def test_golden_fixtures(gpu):
+    """The committed fixtures (tests/golden): every case through the C ABI against its stored
+    Netlib-order result (reltol, verifier, padding rows untouched, NaN padding kept)."""
+    import json
+    import os
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    data = np.load(os.path.join(here, "golden.npz"))
+    with open(os.path.join(here, "golden.json")) as f:
+        meta = json.load(f)
+    for i, c in enumerate(meta):
+        print("case golden %d %s" % (i, c))
+        sys.stdout.flush()
+        A, B, C = data["%d_A" % i], data["%d_B" % i], data["%d_C" % i]
+        st, out = run_gpu(gpu, c["ta"], c["tb"], c["m"], c["n"], c["k"], complex(*c["alpha"]), A, c["lda"], B,
+                          c["ldb"], complex(*c["beta"]), C, c["ldc"])
+        assert st == 0
+        check_against_oracle(out, data["%d_out32" % i], C, c["m"], c["n"], c["ldc"], "golden%d" % i)

@@ -7,6 +7,7 @@ sign-flipped term (brute force, integer-exact inputs), a wrong conj/transpose br
 (reference KATs ZeroProductAllowsNullInputs / EmptyOutputPreservesSentinel).
 """
 import itertools
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -244,3 +245,46 @@ def test_bruteforce_is_exact_rational():
                             np.array([3, 4], np.complex64), 2, 0, np.zeros(1, np.complex64), 1)
     want = Fraction(float(np.float32(0.1))) * 3 + Fraction(float(np.float32(0.2))) * 4
     assert out[0][0] == want and out[0][1] == 0
+
+
+def _golden():
+    import json
+
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    data = np.load(os.path.join(here, "golden.npz"))  # plain arrays, no pickle
+    with open(os.path.join(here, "golden.json")) as f:
+        meta = json.load(f)
+    return data, meta
+
+
+def test_golden_fixtures_reproduce():
+    """The committed fixtures (tests/golden/make_golden.py): the oracle reproduces every stored
+    FP32 result bitwise and FP64 result exactly; the FP64 results agree with numpy's complex128
+    product of the same operands (an independent library routine) to 1e-12 relative; and the
+    FP32 results sit within the FP32 error bound of the FP64 ones."""
+    data, meta = _golden()
+    assert len(meta) == 13
+    for i, c in enumerate(meta):
+        ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+        lda, ldb, ldc = c["lda"], c["ldb"], c["ldc"]
+        alpha, beta = complex(*c["alpha"]), complex(*c["beta"])
+        A, B, C = data["%d_A" % i], data["%d_B" % i], data["%d_C" % i]
+        out32 = oracle.cgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+        assert out32.tobytes() == data["%d_out32" % i].tobytes(), i
+        out64 = oracle.zgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+        assert np.array_equal(out64, data["%d_out64" % i]), i
+        ar, ac = (m, k) if ta == "N" else (k, m)
+        br, bc = (k, n) if tb == "N" else (n, k)
+        opA = oracle.op_matrix(A, ar, ac, lda, ta)
+        opB = oracle.op_matrix(B, br, bc, ldb, tb)
+        al = complex(np.float32(alpha.real), np.float32(alpha.imag))
+        be = complex(np.float32(beta.real), np.float32(beta.imag))
+        ref = al * (opA @ opB)
+        if be != 0:
+            ref = ref + be * C[: ldc * n].reshape(n, ldc).T[:m].astype(np.complex128)
+        got64 = out64.reshape(n, m).T
+        assert np.allclose(got64, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), i
+        got32 = out32[: ldc * n].reshape(n, ldc).T[:m].astype(np.complex128)
+        bound = 4 * (k + 2) * 2.0 ** -24 * (np.abs(al) * (np.abs(opA) @ np.abs(opB)) +
+                                            (np.abs(be) * np.abs(C[: ldc * n].reshape(n, ldc).T[:m]) if be != 0 else 0))
+        assert (np.abs(got32 - ref) <= bound + 1e-30).all(), i
