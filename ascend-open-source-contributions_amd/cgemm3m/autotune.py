@@ -21,6 +21,8 @@ SINGLE_SHAPES = [
     (256, 256, 256), (512, 512, 512), (1024, 1024, 1024), (2048, 2048, 2048), (4096, 4096, 4096),
     (8192, 8192, 8192), (1000, 1500, 2048), (256, 256, 65536), (256, 256, 8192), (512, 512, 16384),
     (128, 128, 4096), (2048, 2048, 256), (4096, 4096, 512), (64, 64, 1024),
+    (64, 64, 64), (128, 128, 128), (128, 128, 512), (384, 384, 384), (512, 512, 128), (768, 768, 768),
+    (1024, 1024, 256), (1024, 1024, 512),
 ]
 GROUPED_SHAPES = [(m, m, k) for m in (32, 64, 128, 256, 512) for k in (64, 256, 1024, 2048)]
 
@@ -66,13 +68,16 @@ def tune_single(h, torch, shapes, reps):
         C = torch.zeros(m * n, dtype=torch.complex64, device="cuda")
         P = cgemm3m.torch_ptr
         res = {}
-        for tile in (64, 128):
-            for parts in (1, 2, 4, 8, 16):
-                if parts > 1 and k // parts < 256:
-                    continue
+        cands = [(tile, parts) for tile in (64, 128) for parts in (1, 2, 4, 8, 16) if parts == 1 or k // parts >= 256]
+        if m * n <= 2048 * 2048:
+            cands.append((32, 1))  # the one-launch small-problem kernel
+        for tile, parts in cands:
+            if tile == 32:
+                cfg = cgemm3m.config("fp32", variant=5)
+            else:
                 cfg = cgemm3m.config("fp32", tile=tile, k_split=parts)
-                res[(tile, parts)] = timed(lambda: cgemm3m.cgemm3m(h, "N", "N", m, n, k, 1.0, P(A), m, P(B), k,
-                                                                   0.0, P(C), m, cfg), reps)
+            res[(tile, parts)] = timed(lambda: cgemm3m.cgemm3m(h, "N", "N", m, n, k, 1.0, P(A), m, P(B), k,
+                                                               0.0, P(C), m, cfg), reps)
         best = choose(res)
         print("single %5d x %5d x %6d  best tile %d parts %2d  %.3f ms  (all: %s)" % (
             m, n, k, best[0], best[1], res[best], {"%d/%d" % kk: round(v, 3) for kk, v in res.items()}),
@@ -120,7 +125,8 @@ def write_table(entries, path=TABLE):
         seen[(mnl, kl, grouped)] = (tile, parts)
     lines = [
         "// tuned_table.h -- plan table written by cgemm3m/autotune.py from MI355X measurements.",
-        "// Key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and K partitions.",
+        "// Key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and K partitions",
+        "// (tile 32: the one-launch small-problem kernel).",
         "// Bins not listed fall back to the planner's occupancy rule.",
         "#pragma once",
         "",

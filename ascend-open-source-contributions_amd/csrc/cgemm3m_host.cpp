@@ -175,8 +175,9 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         tile = cfg->tile;
     } else {
         // The tuned table holds FP32-product measurements; other formats use the occupancy rule.
+        // (An entry naming the small-problem kernel, tile 32, is used further down.)
         int tt = 0, tp = 0;
-        if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp))
+        if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp) && tt != 32)
             tile = tt;
         else
             tile = tiles_for(128) >= cus ? 128 : 64;
@@ -195,8 +196,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
             return false;
         }
         pl.parts = std::min(cfg->kSplit, pl.kSteps);
-    } else if (int tt = 0, tp = 0;
-               pl.format == FMT_F32 && (!cfg || cfg->tile == 0) && tuned_lookup(m, n, k, false, &tt, &tp)) {
+    } else if (int tt = 0, tp = 0; pl.format == FMT_F32 && (!cfg || cfg->tile == 0) &&
+                                   tuned_lookup(m, n, k, false, &tt, &tp) && tt != 32) {
         pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
         // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition.
@@ -213,7 +214,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         return false;
     }
     pl.path = pl.parts > 1 ? 2 : 1;
-    if (cfg && (cfg->variant < 0 || cfg->variant > 4 ||
+    if (cfg && (cfg->variant < 0 || cfg->variant > 5 ||
+                (cfg->variant == 5 && (pl.format != FMT_F32 || pl.identity == ID_FOUR || cfg->kSplit > 1)) ||
                 (cfg->variant == 3 && (pl.format != FMT_F32 || (pl.bm != 128 && pl.identity != ID_FOUR))) ||
                 (cfg->variant == 4 && pl.format != FMT_BF16))) {
         log_error(op, "invalid variant %d for this precision and tile", cfg->variant);
@@ -248,6 +250,31 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.correction = cfg->correction > 0;
     else
         pl.correction = k >= 1024;
+    // One-launch small-problem kernel: no pack, no workspace, no correction. Pinned with
+    // variant 5, or chosen when nothing is pinned and the correction is off, if the tuned
+    // table names it (tile 32) for this shape class or, for classes it does not list, the
+    // output has at most 256 of its 16x16 tiles.
+    {
+        int tt = 0, tp = 0;
+        const bool listed = tuned_lookup(m, n, k, false, &tt, &tp);
+        const bool fits = listed ? tt == 32 : ((m + 15) / 16) * ((n + 15) / 16) <= 256;
+        const bool autoSmall = pl.format == FMT_F32 && !pinned && pl.identity != ID_FOUR && fits;
+        if ((cfg && cfg->variant == 5) || autoSmall) {
+            const int correction = pl.correction;
+            pl = Plan{};
+            pl.precision = ACLBLAS_CGEMM3M_FP32;
+            pl.identity = (cfg && cfg->identity == ID_STANDARD) ? ID_STANDARD : ID_FUSED;
+            pl.path = 5;
+            pl.bm = pl.bn = 16;
+            pl.parts = 1;
+            pl.variant = 4;
+            pl.correction = correction;
+            pl.smallTiles = ((m + 15) / 16) * ((n + 15) / 16);
+            // with the correction on: one flagged-output counter per workgroup
+            pl.total = correction ? (size_t)pl.smallTiles * sizeof(unsigned int) : 0;
+            return true;
+        }
+    }
 
     // Workspace regions.
     const size_t elem = pl.format == FMT_F32 ? 4 : 2;
@@ -322,7 +349,20 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
     hipError_t e = hipSuccess;
     h->lastCounter = nullptr;
     if (pl.path == 0) return ACLBLAS_STATUS_SUCCESS;
-    if (pl.path == 3) {
+    if (pl.path == 5) {
+        const double amag =
+            std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
+        unsigned int* slots = pl.correction ? reinterpret_cast<unsigned int*>(ws) : nullptr;
+        SmallArgs a{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.m, p.n, p.k, pl.identity,
+                    p.alpha.real, p.alpha.imag, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc,
+                    pl.correction, amag * 0x1p-24, 0x1p-8, slots};
+        StageTimer t(h, ST_GEMM);
+        e = launch_gemm_small(a, s);
+        if (slots != nullptr) {
+            h->lastCounter = slots;
+            h->lastCounterSlots = pl.smallTiles;
+        }
+    } else if (pl.path == 3) {
         ScaleCArgs a{p.m, p.n, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc};
         StageTimer t(h, ST_SCALE);
         e = launch_scale_c(a, s);
@@ -464,6 +504,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             StageTimer t(h, ST_RECOMPUTE);
             e = launch_recompute(r, s);
             h->lastCounter = counter;
+            h->lastCounterSlots = 1;
         }
     }
     if (e != hipSuccess) {
@@ -605,13 +646,16 @@ extern "C" aclblasStatus_t aclblasCgemm3mGetLastPlan(aclblasHandle_t handle, int
     if (info == nullptr) return ACLBLAS_STATUS_INVALID_VALUE;
     for (int t = 0; t < 8; ++t) info[t] = handle->lastPlan[t];
     info[6] = -1;
-    if (sync && handle->lastCounter != nullptr) {
-        unsigned int v = 0;
+    if (sync && handle->lastCounter != nullptr && handle->lastCounterSlots > 0) {
+        std::vector<unsigned int> v((size_t)handle->lastCounterSlots, 0u);
         if (hipStreamSynchronize(handle->stream) != hipSuccess ||
-            hipMemcpy(&v, handle->lastCounter, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess) {
+            hipMemcpy(v.data(), handle->lastCounter, v.size() * sizeof(unsigned int), hipMemcpyDeviceToHost) !=
+                hipSuccess) {
             return ACLBLAS_STATUS_EXECUTION_FAILED;
         }
-        info[6] = v;
+        long long total = 0;
+        for (unsigned int x : v) total += x;
+        info[6] = total;
     }
     return ACLBLAS_STATUS_SUCCESS;
 }

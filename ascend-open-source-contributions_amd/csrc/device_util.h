@@ -47,6 +47,18 @@ __device__ __forceinline__ void map_tile(int bid, int nblocks, int tilesM, int t
     tn = w / rows;
 }
 
+// Value of plane q for one complex element (re, im) with the imaginary sign already applied
+// (role 0: A, 1: B).
+__device__ __forceinline__ float plane_value(int role, int identity, int q, float re, float im)
+{
+    if (identity == ID_STANDARD) {
+        return q == 0 ? re : (q == 1 ? im : re + im);
+    }
+    if (identity == ID_FOUR) return q == 0 ? re : (q == 1 ? im : -im);  // A: (Ar, Ai, -Ai); B: (Br, Bi)
+    if (role == 0) return q == 0 ? re + im : (q == 1 ? re : im);  // A: (Ar+Ai, Ar, Ai)
+    return q == 0 ? re : (q == 1 ? im - re : re + im);            // B: (Br, Bi-Br, Br+Bi)
+}
+
 // The three products of one output combined into (re, im).
 __device__ __forceinline__ void combine3(int identity, float p0, float p1, float p2, float& re, float& im)
 {

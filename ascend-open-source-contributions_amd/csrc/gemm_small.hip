@@ -1,0 +1,225 @@
+// gemm_small.hip -- one-launch path for small problems (v_mfma_f32_16x16x4_f32), gfx950.
+//
+// For small outputs the pack + product launches and a long per-wave MFMA chain set the time,
+// not throughput. This kernel reads the caller's interleaved complex64 A and B directly (no
+// workspace, no pack launch) and forms the three 3M plane values in registers. One workgroup
+// computes one 16x16 output tile; its four waves split k (wave w takes the 16-deep k chunks
+// w, w+4, w+8, ...), so a wave's MFMA chain is a quarter of the k range, and the four partial
+// products are added in LDS in wave order (deterministic) before the alpha/beta epilogue.
+// Reference role: the on-chip short-K path (ops-blas/blas/gemm/arch35/cgemm_onchip_kernel.h:
+// 84-353, which deinterleaves into on-chip buffers and needs no workspace), re-designed for
+// MFMA fragments.
+//
+// MFMA operands: the "A" operand comes from op(B) so that output rows i run along the lanes.
+// Within a 16-deep chunk, lane l (t = l/16) holds for MFMA step u the k index
+// 16c + 4t + u, so each lane reads 4 CONSECUTIVE k of its row i of op(A) and its column j of
+// op(B): 32 contiguous bytes when that operand is stored k-contiguous (A as T/C, B as N),
+// and 16-lane runs of 128 contiguous bytes otherwise. The 16x16 result holds output
+// (i0 + l%16, j0 + 4*(l/16) + r) in register r. Out-of-range elements read a clamped
+// in-range address and are replaced by zero, so no load is conditional.
+//
+// Rounding correction in the same launch: the loads that feed the MFMAs also give each row's
+// |op(A)_i|^2 and each column's |op(B)_j|^2 (FP64, summed in a fixed order); an output that
+// fails the screen (as in the other paths) is recomputed by the whole workgroup in FP64 from
+// the inputs (thread t takes k = t, t+256, ...; fixed-order reduction), alpha/beta applied in
+// FP64, rounded once. Flagged outputs are recomputed one after another in whatever order the
+// workgroup listed them; each one's arithmetic is fixed, so the result is deterministic.
+#include "device_util.h"
+#include "launch.h"
+
+namespace c3m {
+
+namespace {
+
+constexpr int kSmallG = 2;  // 16-deep k chunks per register set (4 measured no faster)
+
+struct SmallSet {
+    float2 a[kSmallG][4], b[kSmallG][4];
+};
+
+}  // namespace
+
+__global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int tilesM)
+{
+    __shared__ float red[4][3][256];
+    __shared__ double ered[4][2][16];  // per wave: row energies of op(A), column energies of op(B)
+    __shared__ double dred[4][2];
+    __shared__ int flagged;
+    __shared__ int flagList[256];
+    __shared__ float2 flagC0[256];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tm = blockIdx.x % tilesM, tn = blockIdx.x / tilesM;
+    const int i0 = tm * 16, j0 = tn * 16;
+    const int i = i0 + (lane & 15), j = j0 + (lane & 15), t = lane >> 4;
+    const bool iIn = i < g.m, jIn = j < g.n;
+    const int ic = min(i, g.m - 1), jc = min(j, g.n - 1);
+    const float2* A = reinterpret_cast<const float2*>(g.A);
+    const float2* B = reinterpret_cast<const float2*>(g.B);
+    // op(A)(i, k) = A[i + k*lda] (N) or A[k + i*lda] (T, C); op(B)(k, j) = B[k + j*ldb] (N) or
+    // B[j + k*ldb] (T, C): element offset = fixed part + k * step.
+    const bool aN = g.opA == 111, bN = g.opB == 111;
+    const long long aFix = aN ? ic : (long long)ic * g.lda, aStep = aN ? g.lda : 1;
+    const long long bFix = bN ? (long long)jc * g.ldb : jc, bStep = bN ? 1 : g.ldb;
+    const float aSign = g.opA == 113 ? -1.0f : 1.0f, bSign = g.opB == 113 ? -1.0f : 1.0f;
+    const int K = g.k;
+    const int chunks = (K + 15) / 16;
+    const int iters = (chunks + 3) / 4;  // chunks of this wave: wave, wave + 4, ...
+
+    double ea = 0.0, eb = 0.0;  // this lane's share of |op(A)_i|^2 and |op(B)_j|^2
+    auto load = [&](SmallSet& S, int it) {
+#pragma unroll
+        for (int c = 0; c < kSmallG; ++c)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = ((it * kSmallG + c) * 4 + wave) * 16 + 4 * t + u;
+                const int kc = min(k, K - 1);
+                float2 a = A[aFix + (long long)kc * aStep];
+                float2 b = B[bFix + (long long)kc * bStep];
+                const bool kIn = k < K;
+                a.x = (kIn && iIn) ? a.x : 0.0f;
+                a.y = (kIn && iIn) ? a.y * aSign : 0.0f;
+                b.x = (kIn && jIn) ? b.x : 0.0f;
+                b.y = (kIn && jIn) ? b.y * bSign : 0.0f;
+                S.a[c][u] = a;
+                S.b[c][u] = b;
+            }
+    };
+    f32x4 acc[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    auto mfma = [&](const SmallSet& S) {
+        if (g.correction) {
+#pragma unroll
+            for (int c = 0; c < kSmallG; ++c)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ea += (double)S.a[c][u].x * S.a[c][u].x + (double)S.a[c][u].y * S.a[c][u].y;
+                    eb += (double)S.b[c][u].x * S.b[c][u].x + (double)S.b[c][u].y * S.b[c][u].y;
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < kSmallG; ++c)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        plane_value(1, g.identity, q, S.b[c][u].x, S.b[c][u].y),
+                        plane_value(0, g.identity, q, S.a[c][u].x, S.a[c][u].y), acc[q], 0, 0, 0);
+    };
+
+    // Two register sets of kSmallG chunks; look-ahead loads are unconditional (past the end
+    // they read clamped addresses and yield zeros) and kept ahead of the MFMAs.
+    const int sets = (iters + kSmallG - 1) / kSmallG;
+    SmallSet s0, s1;
+    load(s0, 0);
+    int it = 0;
+    for (; it + 1 < sets; it += 2) {
+        load(s1, it + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma(s0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(s0, it + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma(s1);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (it < sets) mfma(s0);
+
+    // Partial products of the four waves, added in wave order.
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave][q][(lane & 15) + 16 * (4 * t + r)] = acc[q][r];
+    const int tid = threadIdx.x;
+    if (g.correction) {
+        // lanes l, l+16, l+32, l+48 hold the same row i (column j): add them in that order
+        const double ea2 = ea + __shfl_down(ea, 16, 64), eb2 = eb + __shfl_down(eb, 16, 64);
+        const double ea4 = ea2 + __shfl_down(ea2, 32, 64), eb4 = eb2 + __shfl_down(eb2, 32, 64);
+        if (lane < 16) {
+            ered[wave][0][lane] = ea4;
+            ered[wave][1][lane] = eb4;
+        }
+        if (tid == 0) flagged = 0;
+    }
+    __syncthreads();
+    const int io = i0 + (tid & 15), jo = j0 + (tid >> 4);
+    float2* C = reinterpret_cast<float2*>(g.C);
+    const bool bz = g.betaZero != 0;
+    if (io < g.m && jo < g.n) {
+        float p[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) p[q] = ((red[0][q][tid] + red[1][q][tid]) + red[2][q][tid]) + red[3][q][tid];
+        float re, im;
+        combine3(g.identity, p[0], p[1], p[2], re, im);
+        const size_t o = (size_t)jo * g.ldc + io;
+        float2 c0 = make_float2(0.0f, 0.0f);
+        if (!bz) c0 = C[o];
+        const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0);
+        bool flag = false;
+        if (g.correction) {
+            const int il = tid & 15, jl = tid >> 4;
+            const double eA = ((ered[0][0][il] + ered[1][0][il]) + ered[2][0][il]) + ered[3][0][il];
+            const double eB = ((ered[0][1][jl] + ered[1][1][jl]) + ered[2][1][jl]) + ered[3][1][jl];
+            flag = needs_recompute(c, eA, eB, g.corrScale, g.corrTau);
+        }
+        if (flag) {
+            const int slot = atomicAdd(&flagged, 1);
+            flagList[slot] = tid;
+            flagC0[slot] = c0;
+        } else {
+            C[o] = c;
+        }
+    }
+    if (!g.correction) return;
+    __syncthreads();
+    const int count = flagged;
+    for (int e = 0; e < count; ++e) {
+        const int ft = flagList[e];
+        const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);
+        double sr = 0.0, si = 0.0;
+        for (long long l = tid; l < K; l += 256) {
+            const float2 x = aN ? A[(size_t)l * g.lda + fi] : A[(size_t)fi * g.lda + l];  // op(A)(fi, l)
+            const float2 y = bN ? B[(size_t)fj * g.ldb + l] : B[(size_t)l * g.ldb + fj];  // op(B)(l, fj)
+            const double xr = x.x, xi = aSign * (double)x.y, yr = y.x, yi = bSign * (double)y.y;
+            sr = fma(xr, yr, sr);
+            sr = fma(-xi, yi, sr);
+            si = fma(xr, yi, si);
+            si = fma(xi, yr, si);
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            sr += __shfl_xor(sr, d, 64);
+            si += __shfl_xor(si, d, 64);
+        }
+        if (lane == 0) {
+            dred[wave][0] = sr;
+            dred[wave][1] = si;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double tr = ((dred[0][0] + dred[1][0]) + dred[2][0]) + dred[3][0];
+            const double ti = ((dred[0][1] + dred[1][1]) + dred[2][1]) + dred[3][1];
+            double cr = g.alphaR * tr - g.alphaI * ti;
+            double ci = g.alphaR * ti + g.alphaI * tr;
+            if (!bz) {
+                const float2 c0 = flagC0[e];
+                cr += g.betaR * c0.x - g.betaI * c0.y;
+                ci += g.betaR * c0.y + g.betaI * c0.x;
+            }
+            C[(size_t)fj * g.ldc + fi] = make_float2((float)cr, (float)ci);
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && g.flagCount != nullptr) g.flagCount[blockIdx.x] = (unsigned int)count;
+}
+
+hipError_t launch_gemm_small(const SmallArgs& g, hipStream_t s)
+{
+    const int tilesM = (g.m + 15) / 16, tilesN = (g.n + 15) / 16;
+    hipLaunchKernelGGL(gemm3m_small_kernel, dim3(tilesM * tilesN), dim3(256), 0, s, g, tilesM);
+    return hipGetLastError();
+}
+
+}  // namespace c3m

@@ -18,7 +18,8 @@ struct _aclblas_handle {
     int device = -1;            // device the default workspace lives on
     int cus = 0;                // compute units of that device
     long long lastPlan[8] = {0, 0, 0, 0, 0, 0, -1, 0};
-    unsigned int* lastCounter = nullptr;  // device counter of flagged outputs (last call)
+    unsigned int* lastCounter = nullptr;  // device counters of flagged outputs (last call),
+    long long lastCounterSlots = 0;       // summed by aclblasCgemm3mGetLastPlan
     int timing = 0;                       // aclblasCgemm3mSetTiming
     struct Timed {
         int stage;
@@ -73,6 +74,7 @@ struct Plan {
     int precision = 0;    // aclblasCgemm3mPrecision_t
     int format = FMT_F32;
     int pieces = 1;
+    long long smallTiles = 0;  // path 5: 16x16 output tiles (one flagged-output slot each)
     int bm = 128, bn = 128;
     int bk = 16;          // k depth of one GEMM stage
     int parts = 1;

@@ -91,6 +91,18 @@ struct GemmArgs {
     void* shadow;                           // beta != 0: original C of flagged outputs, ld = m
 };
 
+struct SmallArgs {      // one-launch path for small problems: complex inputs read directly
+    const void* A; long long lda; int opA;  // op: 111 N, 112 T, 113 C
+    const void* B; long long ldb; int opB;
+    int m, n, k;
+    int identity;
+    float alphaR, alphaI, betaR, betaI; int betaZero;
+    void* C; long long ldc;
+    int correction;                         // screen + in-kernel FP64 recompute of flagged outputs
+    double corrScale, corrTau;
+    unsigned int* flagCount;                // optional: flagged outputs per workgroup (one slot each)
+};
+
 struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/beta
     const float* partial; size_t partStride; int parts; int ldp;
     int products;       // 1: partial holds the three real products P0, P1, P2 (one plane each)

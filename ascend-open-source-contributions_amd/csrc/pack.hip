@@ -17,17 +17,6 @@
 
 namespace c3m {
 
-// Value of plane q for one complex element (re, im) with the imaginary sign already applied.
-__device__ __forceinline__ float plane_value(int role, int identity, int q, float re, float im)
-{
-    if (identity == ID_STANDARD) {
-        return q == 0 ? re : (q == 1 ? im : re + im);
-    }
-    if (identity == ID_FOUR) return q == 0 ? re : (q == 1 ? im : -im);  // A: (Ar, Ai, -Ai); B: (Br, Bi)
-    if (role == 0) return q == 0 ? re + im : (q == 1 ? re : im);  // A: (Ar+Ai, Ar, Ai)
-    return q == 0 ? re : (q == 1 ? im - re : re + im);            // B: (Br, Bi-Br, Br+Bi)
-}
-
 __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, float2 (*tile)[65], double* esum)
 {
     const int tid = threadIdx.x;

@@ -143,7 +143,9 @@ typedef struct {
     int variant;    /* 0 auto; 128x128 tiles: 1 four waves through LDS, 2 eight waves through
                        LDS (two per SIMD), 3 four waves loading fragments straight into
                        registers, no LDS (FP32 only); 4 separate products: each real product
-                       on its own 256x256 tiles, then a combine pass (bf16x3/bf16x6 only) */
+                       on its own 256x256 tiles, then a combine pass (bf16x3/bf16x6 only);
+                       5 one-launch small-problem kernel reading A and B directly, no
+                       workspace (FP32 3M, no K split, correction off) */
 } aclblasCgemm3mConfig;
 
 /* For callers that choose the arithmetic or pin the plan (bench, tests, autotuner, the

@@ -114,7 +114,9 @@ def test_validation_order(h):
 
 def test_config_validation(h):
     for cfg in (cgemm3m.config(precision=9), cgemm3m.config(tile=96), cgemm3m.config(k_split=17),
-                cgemm3m.config(correction=2), cgemm3m.config(identity=5), cgemm3m.config(variant=5), cgemm3m.config(variant=4),
+                cgemm3m.config(correction=2), cgemm3m.config(identity=5), cgemm3m.config(variant=6), cgemm3m.config(variant=4),
+                cgemm3m.config("bf16x3", variant=5), cgemm3m.config(variant=5, k_split=2),
+                cgemm3m.config(variant=5, identity=3),
                 cgemm3m.config("bf16x3", variant=4, k_split=2), cgemm3m.config("bf16x3", variant=4, tile=64),
                 cgemm3m.config("bf16x3", variant=3), cgemm3m.config(tile=64, variant=3)):
         st = cgemm3m.cgemm3m(h, "N", "N", 16, 16, 16, 1, FAKE, 16, FAKE, 16, 0, FAKE, 16, cfg, check=False)
@@ -122,9 +124,11 @@ def test_config_validation(h):
 
 
 def test_workspace_size_query(h):
-    n = cgemm3m.workspace_size(h, "N", "N", 256, 256, 256, 1, 0)
-    # 3 A planes + 3 B planes of 256 x 256 FP32 at least
-    assert n >= 6 * 256 * 256 * 4
+    n = cgemm3m.workspace_size(h, "N", "N", 1024, 1024, 1024, 1, 0)
+    # 3 A planes + 3 B planes of 1024 x 1024 FP32 at least
+    assert n >= 6 * 1024 * 1024 * 4
+    # the one-launch small-problem kernel reads A and B directly: no workspace
+    assert cgemm3m.workspace_size(h, "N", "N", 100, 100, 100, 1, 0) == 0
     big = cgemm3m.workspace_size(h, "N", "N", 8192, 8192, 8192, 1, 0.5)
     assert big >= 6 * 8192 * 8192 * 4
     bf = cgemm3m.workspace_size(h, "N", "N", 8192, 8192, 8192, 1, 0, cgemm3m.config("bf16x3"))

@@ -331,3 +331,29 @@ def test_golden_fixtures(gpu):
                           c["ldb"], complex(*c["beta"]), C, c["ldc"])
         assert st == 0
         check_against_oracle(out, data["%d_out32" % i], C, c["m"], c["n"], c["ldc"], "golden%d" % i)
+
+
+@pytest.mark.parametrize("ta,tb", list(itertools.product("NTC", "NTC")))
+@pytest.mark.parametrize("identity", [1, 2])
+def test_small_kernel_ops(gpu, ta, tb, identity):
+    """The one-launch small-problem kernel (variant 5: reads A and B directly, 16x16 output per
+    wave) on ragged shapes with padded leading dimensions, every op pair, both identities."""
+    import cgemm3m
+
+    m, n, k = 100, 70, 91
+    lda = (m if ta == "N" else k) + 3
+    ldb = (k if tb == "N" else n) + 2
+    cfg = cgemm3m.config(variant=5, identity=identity)
+    out, plan = _case(gpu, ta, tb, m, n, k, ALPHA2, BETA2, lda, ldb, m + 5,
+                      20260800 + 9 * identity + 3 * "NTC".index(ta) + "NTC".index(tb), cfg=cfg, label="small")
+    assert plan["path"] == 5
+
+
+@pytest.mark.parametrize("shape", [(200, 150, 300), (1, 1, 1), (33, 17, 1000), (64, 300, 5)])
+def test_small_kernel_auto(gpu, shape):
+    """Shapes the planner sends to the small-problem kernel by default (correction off,
+    beta == 0 included)."""
+    m, n, k = shape
+    for beta in (BETA2, 0):
+        out, plan = _case(gpu, "N", "C", m, n, k, ALPHA2, beta, m, n, m, 20260850 + m + n + k, label="small-auto")
+        assert plan["path"] == 5
