@@ -214,7 +214,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         return false;
     }
     pl.path = pl.parts > 1 ? 2 : 1;
-    if (cfg && (cfg->variant < 0 || cfg->variant > 5 ||
+    if (cfg && (cfg->variant < 0 || cfg->variant > 6 ||
+                (cfg->variant == 6 && pl.format != FMT_BF16) ||
                 (cfg->variant == 5 && (pl.format != FMT_F32 || pl.identity == ID_FOUR || cfg->kSplit > 1)) ||
                 (cfg->variant == 3 && (pl.format != FMT_F32 || (pl.bm != 128 && pl.identity != ID_FOUR))) ||
                 (cfg->variant == 4 && pl.format != FMT_BF16))) {
@@ -229,7 +230,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     // 577, bf16x6 232 vs 319; 2048^3: 291 vs 323, 173 vs 198).
     const bool pinned = cfg && (cfg->tile != 0 || cfg->kSplit != 0 || cfg->variant != 0);
     const long long tiles256 = ((m + 255) / 256) * ((n + 255) / 256);
-    const bool products = (cfg && cfg->variant == 4) ||
+    const bool products = (cfg && (cfg->variant == 4 || cfg->variant == 6)) ||
                           (pl.format == FMT_BF16 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2);
     if (products) {
         if (cfg && (cfg->kSplit > 1 || (cfg->tile != 0 && cfg->tile != 128))) {
@@ -241,6 +242,14 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.parts = 1;
         pl.mPad = (int)round_up((size_t)m, 256);
         pl.nPad = (int)round_up((size_t)n, 256);
+        // kernel: the register-prefetch form by default; variant 6 pins the LDS-DMA form (byte-
+        // identical, measured slower: bf16x3 at 8192^3 7.52 ms against 6.94 ms), whose unrolled
+        // stage loop wants k in whole groups of 4 (bf16x3) or 6 (bf16x6) 16-deep stages
+        pl.variant = (cfg && cfg->variant == 6) ? 5 : 3;
+        if (pl.variant == 5) {
+            pl.kPad = (int)round_up((size_t)k, 192);
+            pl.kSteps = pl.kPad / pl.bk;
+        }
     }
     if (cfg && (cfg->correction < -1 || cfg->correction > 1)) {
         log_error(op, "invalid correction flag %d", cfg->correction);
@@ -444,7 +453,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         if (e == hipSuccess) {
             StageTimer t(h, ST_GEMM);
             if (pl.path == 4)
-                e = launch_gemm_bf16_products(g, pl.pieces, s);
+                e = launch_gemm_bf16_products(g, pl.pieces, pl.variant == 5, s);
             else if (pl.format == FMT_F32)
                 e = launch_gemm_f32(g, pl.bm, pl.bn, pl.variant, s);
             else if (pl.format == FMT_BF16)

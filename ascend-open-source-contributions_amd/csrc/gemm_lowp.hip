@@ -321,11 +321,138 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_kernel(GemmArgs g
             }
 }
 
+// LDS-DMA form of the separate-products kernel. The workgroup's four waves load each stage's
+// fragments ONCE per workgroup (instead of once per wave that needs them) with direct
+// global->LDS loads (global_load_lds_dwordx4: 64 lanes x 16 B = one 1 KiB fragment per
+// instruction, no register staging), into a ring of R stage slots, so R-1 stages are in flight
+// while a stage computes. Per stage t: wait for this wave's loads of stage t+1 and for its own
+// LDS reads, barrier (all waves' stage-t+1 data in LDS, all reads of stage t's slot done),
+// issue the loads of stage t+R into stage t's slot, read stage t+1's fragments into the other
+// register set, then the MFMAs of stage t. k is padded to a multiple of 16*U (planner), so the
+// unrolled loop needs no tail. Same MFMA order per accumulator as the other forms.
+template <int NP, int R>
+__global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmArgs g)
+{
+    constexpr int TM = 4, TN = 4;
+    constexpr int NTERM = NP == 2 ? 3 : 6;
+    constexpr int TB[6] = {1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
+    constexpr int TA[6] = {NP == 2 ? 0 : 1, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
+    constexpr int SLOT = 16 * NP * 64;                // u32x4 per stage: 8*NP A + 8*NP B fragments
+    constexpr int U = (R % 2 == 0) ? R : 2 * R;       // unroll: slot and register set compile-time
+    constexpr int PER = 4 * NP;                       // loads per wave per stage
+    constexpr int VM = (R - 2) * PER;                 // loads left in flight when stage t+1 is needed
+    constexpr int WAIT = (VM & 15) | ((VM >> 4) << 14) | (7 << 4);  // vmcnt(VM), expcnt max, lgkmcnt(0)
+    static_assert(VM < 64, "vmcnt holds at most 63");
+    __shared__ u32x4 lds[R * SLOT];
+
+    int q, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, q, tmI, tnI);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const size_t aRowB = (size_t)(g.mPad / 32) * 1024;
+    const size_t bRowB = (size_t)(g.nPad / 32) * 1024;
+    const char* aBlk = reinterpret_cast<const char*>(g.Ap) + (size_t)(q * NP) * g.aPlaneBytes + (size_t)tmI * 8 * 1024;
+    const char* bBlk = reinterpret_cast<const char*>(g.Bp) + (size_t)(q * NP) * g.bPlaneBytes + (size_t)tnI * 8 * 1024;
+    const unsigned laneB = (unsigned)lane * 16u;
+    const int nst = g.kSteps;
+
+    // wave w loads mn fragments 2w and 2w+1 of A and of B, every piece
+    auto issue = [&](int slot, int ks) {
+        u32x4* L = lds + slot * SLOT;
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                const int fr = 2 * wave + f;
+                __builtin_amdgcn_global_load_lds(
+                    aBlk + (size_t)p * g.aPlaneBytes + (size_t)ks * aRowB + fr * 1024 + laneB,
+                    (__attribute__((address_space(3))) void*)(L + (p * 8 + fr) * 64), 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(
+                    bBlk + (size_t)p * g.bPlaneBytes + (size_t)ks * bRowB + fr * 1024 + laneB,
+                    (__attribute__((address_space(3))) void*)(L + (8 * NP + p * 8 + fr) * 64), 16, 0, 0);
+            }
+    };
+    struct Set {
+        u32x4 a[NP][TM], b[NP][TN];
+    };
+    auto read = [&](Set& S, int slot) {
+        const u32x4* L = lds + slot * SLOT;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int i = 0; i < TM; ++i) S.a[p][i] = L[(p * 8 + wm * TM + i) * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) S.b[p][j] = L[(8 * NP + p * 8 + wn * TN + j) * 64 + lane];
+        }
+    };
+    f32x16 acc[TN][TM];
+#pragma unroll
+    for (int a = 0; a < TN; ++a)
+#pragma unroll
+        for (int b = 0; b < TM; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+    auto mfma = [&](const Set& S) {
+#pragma unroll
+        for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+            for (int a = 0; a < TN; ++a)
+#pragma unroll
+                for (int b = 0; b < TM; ++b) acc[a][b] = mfma16(S.b[TB[t]][a], S.a[TA[t]][b], acc[a][b], false);
+    };
+
+    Set S[2];
+#pragma unroll
+    for (int u = 0; u < R; ++u) issue(u, min(u, nst - 1));
+    __builtin_amdgcn_s_waitcnt((((R - 1) * PER) & 15) | ((((R - 1) * PER) >> 4) << 14) | (7 << 4) | (15 << 8));
+    __builtin_amdgcn_s_barrier();
+    read(S[0], 0);
+    for (int s0 = 0; s0 < nst; s0 += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            __builtin_amdgcn_s_waitcnt(WAIT);
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+            issue(u % R, min(s0 + u + R, nst - 1));
+            read(S[(u + 1) & 1], (u + 1) % R);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma(S[u & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // no load may still be writing LDS when the workgroup ends
+    __builtin_amdgcn_s_waitcnt(0);
+
+    float* P = g.partial + (size_t)q * g.partStride;
+    const int iBase = tmI * 256 + wm * 128 + (lane & 31);
+    const int jBase = tnI * 256 + wn * 128 + 4 * (lane >> 5);
+#pragma unroll
+    for (int a = 0; a < TN; ++a)
+#pragma unroll
+        for (int b = 0; b < TM; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = iBase + b * 32;
+                const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+                P[(size_t)j * g.mPad + i] = acc[a][b][r];
+            }
+}
+
 // g.tilesM / g.tilesN count 256 x 256 tiles (mPad, nPad multiples of 256); g.partial holds
 // three FP32 planes of mPad x nPad, g.partStride floats apart; the whole k range (no split).
-hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, hipStream_t s)
+hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s)
 {
     const int blocks = 3 * g.tilesM * g.tilesN;
+    if (lds) {
+        if (pieces == 2)
+            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<2, 4>), dim3(blocks), dim3(256), 0, s, g);
+        else if (pieces == 3)
+            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<3, 3>), dim3(blocks), dim3(256), 0, s, g);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     if (pieces == 2)
         hipLaunchKernelGGL((gemm3m_lowp_products_kernel<2, 3>), dim3(blocks), dim3(256), 0, s, g);
     else if (pieces == 3)

@@ -12,7 +12,7 @@ hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s);
 hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, int variant, hipStream_t s);
-hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, hipStream_t s);
+hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s);
 hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, hipStream_t s);
 hipError_t launch_gemm_small(const SmallArgs& g, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);

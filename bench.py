@@ -44,8 +44,9 @@ def main():
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-run", action="store_true",
-                    help="for rocprofv3 counter passes: 3 plain fp32 NN calls, then one device copy "
-                         "of exactly 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE), nothing timed")
+                    help="for rocprofv3 counter passes: 3 NN calls of the first --variants form (fp32 "
+                         "by default), then one device copy of exactly 1 GiB (calibrates FETCH_SIZE/"
+                         "WRITE_SIZE), nothing timed")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -79,14 +80,16 @@ def main():
         return cgemm3m.cgemm3m(h, ta, tb, m, n, k, alpha, P(Ause), m, P(B), m, beta, P(Cbuf), m, cfg, check=False)
 
     if args.profile_run:
+        # the first of --variants (fp32 unless --variants is given)
+        form = args.variants.split(",")[0] if args.variants != ap.get_default("variants") else "fp32"
         C = C0.clone()
         for _ in range(3):
-            call(cgemm3m.config("fp32"), "NN", C)
+            call(cgemm3m.config(form), "NN", C)
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        log("profile run done: 3 x fp32 NN %d^3, then a 1 GiB copy" % m)
+        log("profile run done: 3 x %s NN %d^3, then a 1 GiB copy" % (form, m))
         return
 
     results = {}
