@@ -138,10 +138,40 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC
                     pi[o] = im;
                 }
     } else {
+        // One 32-column slice (a) at a time: every load the slice needs (old C when beta != 0,
+        // the row and column energies when the screen is on) is issued first, from clamped
+        // in-range addresses, and waited for once; a load inside the per-output branch would
+        // cost one full memory round trip per output.
         float2* C = reinterpret_cast<float2*>(g.C);
         const bool bz = g.betaZero != 0;
+        double ea[TM];
+        if (g.correction) {
 #pragma unroll
-        for (int a = 0; a < TN; ++a)
+            for (int b = 0; b < TM; ++b) ea[b] = g.eA[iBase + b * 32];  // eA holds mPad rows
+        }
+#pragma unroll
+        for (int a = 0; a < TN; ++a) {
+            float2 c0[TM][16];
+            double eb[16];
+            if (!bz) {
+#pragma unroll
+                for (int b = 0; b < TM; ++b)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int ic = min(iBase + b * 32, g.m - 1);
+                        const int jc = min(jBase + a * 32 + 8 * (r >> 2) + (r & 3), g.n - 1);
+                        c0[b][r] = C[(size_t)jc * g.ldc + ic];
+                    }
+            } else {
+#pragma unroll
+                for (int b = 0; b < TM; ++b)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) c0[b][r] = make_float2(0.0f, 0.0f);
+            }
+            if (g.correction) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) eb[r] = g.eB[jBase + a * 32 + 8 * (r >> 2) + (r & 3)];  // nPad columns
+            }
 #pragma unroll
             for (int b = 0; b < TM; ++b)
 #pragma unroll
@@ -152,17 +182,16 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC
                         float re, im;
                         combine(a, b, r, re, im);
                         const size_t o = (size_t)j * g.ldc + i;
-                        float2 c0 = make_float2(0.0f, 0.0f);
-                        if (!bz) c0 = C[o];
-                        const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0);
-                        if (g.correction && needs_recompute(c, g.eA[i], g.eB[j], g.corrScale, g.corrTau)) {
+                        const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0[b][r]);
+                        if (g.correction && needs_recompute(c, ea[b], eb[r], g.corrScale, g.corrTau)) {
                             const long long lin = (long long)j * g.m + i;
                             flag_output(g.bitmap, lin);
-                            if (!bz) reinterpret_cast<float2*>(g.shadow)[lin] = c0;
+                            if (!bz) reinterpret_cast<float2*>(g.shadow)[lin] = c0[b][r];
                         }
                         C[o] = c;
                     }
                 }
+        }
     }
 }
 

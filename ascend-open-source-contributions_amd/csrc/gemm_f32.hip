@@ -179,24 +179,36 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_f32_kernel(GemmArgs g)
 // FOUR = true is the four-product form (ID_FOUR): A planes (Ar, Ai, -Ai), B planes (Br, Bi),
 // two accumulators per 32x32 tile taking re += Br*Ar then Bi*(-Ai), im += Bi*Ar then Br*Ai
 // per k slot; the epilogue stores them as (re, im).
-template <int BM, int BN, int WM, int WN, int NB>
+//
+// KFD: 8-deep k fragments per register set (stage); NSET: register sets in the ring, so the
+// loads of stage s + NSET - 1 are issued before the MFMAs of stage s.
+#ifndef C3M_DIRECT_KFD
+#define C3M_DIRECT_KFD 2
+#endif
+#ifndef C3M_DIRECT_NSET
+#define C3M_DIRECT_NSET 2
+#endif
+template <int BM, int BN, int WM, int WN, int NB, int KFD>
 struct DirectSet {
     using T = F32Tile<BM, BN, WM, WN>;
-    f32x4 a[T::KF][3][T::TM], b[T::KF][NB][T::TN];
+    f32x4 a[KFD][3][T::TM], b[KFD][NB][T::TN];
 };
 
-template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2, bool FOUR = false>
+template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2, bool FOUR = false, int KFD = C3M_DIRECT_KFD,
+          int NSET = C3M_DIRECT_NSET>
 __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int tmI, int tnI)
 {
     using T = F32Tile<BM, BN, WM, WN>;
     constexpr int NB = FOUR ? 2 : 3;    // B planes
     constexpr int NACC = FOUR ? 2 : 3;  // accumulators per 32x32 output tile
-    using Set = DirectSet<BM, BN, WM, WN, NB>;
+    static_assert(KFD == 1 || KFD == 2, "a planner stage (16 k) is two 8-deep fragments");
+    using Set = DirectSet<BM, BN, WM, WN, NB, KFD>;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave % WM, wn = wave / WM;
-    const int ks0 = (int)((long long)g.kSteps * part / g.parts);
-    const int ks1 = (int)((long long)g.kSteps * (part + 1) / g.parts);
+    // this partition's k fragments [kf0, kf0 + KFD * nst)
+    const int kf0 = 2 * (int)((long long)g.kSteps * part / g.parts);
+    const int kf1 = 2 * (int)((long long)g.kSteps * (part + 1) / g.parts);
 
     // Wave-uniform byte addresses (scalar registers) plus one 32-bit lane offset, so every
     // load is a scalar base + vector offset form and addressing costs almost no VGPRs.
@@ -206,13 +218,14 @@ __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int
     const char* bW = reinterpret_cast<const char*>(g.Bp) + (size_t)(tnI * T::FB + wn * T::TN) * 1024;
     const unsigned laneB = (unsigned)lane * 16u;
 
-    auto load = [&](Set& S, int ks) {
+    auto load = [&](Set& S, int st) {
 #pragma unroll
-        for (int kfl = 0; kfl < T::KF; ++kfl)
+        for (int kfl = 0; kfl < KFD; ++kfl)
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                const char* a = aW + (size_t)q * g.aPlaneBytes + (size_t)(ks * T::KF + kfl) * aRowB;
-                const char* b = bW + (size_t)q * g.bPlaneBytes + (size_t)(ks * T::KF + kfl) * bRowB;
+                const size_t kf = (size_t)(kf0 + st * KFD + kfl);
+                const char* a = aW + (size_t)q * g.aPlaneBytes + kf * aRowB;
+                const char* b = bW + (size_t)q * g.bPlaneBytes + kf * bRowB;
 #pragma unroll
                 for (int i = 0; i < T::TM; ++i)
                     S.a[kfl][q][i] = *reinterpret_cast<const f32x4*>(a + i * 1024 + laneB);
@@ -238,7 +251,7 @@ __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int
     constexpr int TACC[4] = {0, 1, 0, 1}, TBP[4] = {0, 1, 1, 0}, TAP[4] = {0, 0, 2, 1};
     auto mfma = [&](const Set& S) {
 #pragma unroll
-        for (int kfl = 0; kfl < T::KF; ++kfl)
+        for (int kfl = 0; kfl < KFD; ++kfl)
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 if constexpr (FOUR) {
@@ -263,26 +276,30 @@ __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int
             }
     };
 
-    // Stages in pairs (set 0, set 1) with one exit, so the accumulators keep their registers;
-    // an odd last stage is already loaded into set 0.
-    // Loads are unconditional (the last pair's look-ahead is clamped to the last stage): a
-    // load skipped on one path would make the wait before the next MFMAs cover the loads just
+    // Ring of NSET register sets, stages in groups of NSET with one exit, so the accumulators
+    // keep their registers; the last NSET - 1 stages are already loaded when the loop ends.
+    // Loads are unconditional (look-ahead past the end is clamped to the last stage): a load
+    // skipped on one path would make the wait before the next MFMAs cover the loads just
     // issued. Scheduling barriers keep each stage's loads ahead of the MFMAs.
-    Set s0, s1;
-    const int nst = ks1 - ks0;
-    if (nst > 0) load(s0, ks0);
-    int st = 0;
-    for (; st + 1 < nst; st += 2) {
-        load(s1, ks0 + st + 1);
-        __builtin_amdgcn_sched_barrier(0);
-        mfma(s0);
-        __builtin_amdgcn_sched_barrier(0);
-        load(s0, ks0 + min(st + 2, nst - 1));
-        __builtin_amdgcn_sched_barrier(0);
-        mfma(s1);
-        __builtin_amdgcn_sched_barrier(0);
+    const int nst = (kf1 - kf0) / KFD;
+    Set S[NSET];
+    if (nst > 0) {
+#pragma unroll
+        for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nst - 1));
     }
-    if (st < nst) mfma(s0);
+    int st = 0;
+    for (; st + NSET <= nst; st += NSET) {
+#pragma unroll
+        for (int u = 0; u < NSET; ++u) {
+            load(S[(u + NSET - 1) % NSET], min(st + u + NSET - 1, nst - 1));
+            __builtin_amdgcn_sched_barrier(0);
+            mfma(S[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u + 1 < NSET; ++u)
+        if (st + u < nst) mfma(S[u]);
 
     const int iBase = tmI * BM + wm * (BM / WM) + (lane & 31);
     const int jBase = tnI * BN + wn * (BN / WN) + 4 * (lane >> 5);
