@@ -251,12 +251,13 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
             pl.kSteps = pl.kPad / pl.bk;
         }
     }
-    if (cfg && (cfg->correction < -1 || cfg->correction > 1)) {
+    if (cfg && (cfg->correction < -1 || cfg->correction > 2)) {
         log_error(op, "invalid correction flag %d", cfg->correction);
         return false;
     }
+    // pl.correction: 0 off, 1 FP64 recompute, 2 recompute in the reference BLAS's FP32 order
     if (cfg && cfg->correction != 0)
-        pl.correction = cfg->correction > 0;
+        pl.correction = cfg->correction > 0 ? cfg->correction : 0;
     else
         pl.correction = k >= 1024;
     // One-launch small-problem kernel: no pack, no workspace, no correction. Pinned with
@@ -510,6 +511,11 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             r.C = p.C;
             r.ldc = p.ldc;
             r.counter = counter;
+            r.netlib = pl.correction == 2 ? 1 : 0;
+            r.alphaRf = p.alpha.real;
+            r.alphaIf = p.alpha.imag;
+            r.betaRf = p.beta.real;
+            r.betaIf = p.beta.imag;
             StageTimer t(h, ST_RECOMPUTE);
             e = launch_recompute(r, s);
             h->lastCounter = counter;

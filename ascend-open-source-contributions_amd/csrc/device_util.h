@@ -47,6 +47,56 @@ __device__ __forceinline__ void map_tile(int bid, int nblocks, int tilesM, int t
     tn = w / rows;
 }
 
+// One output recomputed in the reference BLAS's (Netlib cgemm's) FP32 order, every complex
+// product the naive (ar*br - ai*bi, ar*bi + ai*br) with one rounding per operation:
+//   op(A) = N: c = beta*c0 (0 when beta == 0, c0 when beta == 1); for l: c += (alpha*op(B)(l,j))*A(i,l)
+//   op(A) = T/C: t = sum_l op(A)(i,l)*op(B)(l,j) in l order; c = alpha*t (+ beta*c0)
+// (Netlib reference BLAS 3.10.0 cgemm loop nests, the oracle the reference checks against,
+// ops-blas/test/gemm/cgemm3m/cgemm3m_golden.h:46-48.)
+__device__ __forceinline__ float2 netlib_cmul(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 netlib_output(const float2* A, long long lda, int opA, const float2* B, long long ldb,
+                                                int opB, long long i, long long j, long long k, float2 alpha,
+                                                float2 beta, bool betaZero, float2 c0)
+{
+#pragma clang fp contract(off)
+    auto opb = [&](long long l) {
+        float2 y = opB == 111 ? B[(size_t)j * ldb + l] : B[(size_t)l * ldb + j];
+        if (opB == 113) y.y = -y.y;
+        return y;
+    };
+    const bool betaOne = beta.x == 1.0f && beta.y == 0.0f;
+    if (opA == 111) {
+        float2 c = betaZero ? make_float2(0.0f, 0.0f) : (betaOne ? c0 : netlib_cmul(beta, c0));
+        for (long long l = 0; l < k; ++l) {
+            const float2 t = netlib_cmul(alpha, opb(l));
+            const float2 p = netlib_cmul(t, A[(size_t)l * lda + i]);
+            c.x = c.x + p.x;
+            c.y = c.y + p.y;
+        }
+        return c;
+    }
+    float2 t = make_float2(0.0f, 0.0f);
+    for (long long l = 0; l < k; ++l) {
+        float2 x = A[(size_t)i * lda + l];
+        if (opA == 113) x.y = -x.y;
+        const float2 p = netlib_cmul(x, opb(l));
+        t.x = t.x + p.x;
+        t.y = t.y + p.y;
+    }
+    float2 c = netlib_cmul(alpha, t);
+    if (!betaZero) {
+        const float2 q = netlib_cmul(beta, c0);
+        c.x = c.x + q.x;
+        c.y = c.y + q.y;
+    }
+    return c;
+}
+
 // Value of plane q for one complex element (re, im) with the imaginary sign already applied
 // (role 0: A, 1: B).
 __device__ __forceinline__ float plane_value(int role, int identity, int q, float re, float im)

@@ -130,6 +130,22 @@ __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
     const float2* A = reinterpret_cast<const float2*>(a.A);
     const float2* B = reinterpret_cast<const float2*>(a.B);
     float2* C = reinterpret_cast<float2*>(a.C);
+    if (a.netlib) {
+        // one flagged output per lane, each in the reference BLAS's sequential FP32 order
+        for (long long w = wave; w < a.words; w += waves) {
+            const unsigned long long bits = a.bitmap[w];
+            if ((bits >> lane) & 1ull) {
+                const long long lin = w * 64 + lane;
+                const long long i = lin % a.m, j = lin / a.m;
+                const float2 c0 = a.betaZero ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.shadow)[lin];
+                C[(size_t)j * a.ldc + i] =
+                    netlib_output(A, a.lda, a.opA, B, a.ldb, a.opB, i, j, a.k, make_float2(a.alphaRf, a.alphaIf),
+                                  make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
+                if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+            }
+        }
+        return;
+    }
     for (long long w = wave; w < a.words; w += waves) {
         unsigned long long bits = a.bitmap[w];
         while (bits != 0ull) {

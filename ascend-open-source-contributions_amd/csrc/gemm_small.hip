@@ -175,6 +175,18 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     if (!g.correction) return;
     __syncthreads();
     const int count = flagged;
+    if (g.correction == 2) {
+        // the reference BLAS's sequential FP32 order: one flagged output per thread
+        for (int e = tid; e < count; e += 256) {
+            const int ft = flagList[e];
+            const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);
+            C[(size_t)fj * g.ldc + fi] = netlib_output(A, g.lda, g.opA, B, g.ldb, g.opB, fi, fj, K,
+                                                       make_float2(g.alphaR, g.alphaI), make_float2(g.betaR, g.betaI),
+                                                       bz, flagC0[e]);
+        }
+        if (tid == 0 && g.flagCount != nullptr) g.flagCount[blockIdx.x] = (unsigned int)count;
+        return;
+    }
     for (int e = 0; e < count; ++e) {
         const int ft = flagList[e];
         const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);

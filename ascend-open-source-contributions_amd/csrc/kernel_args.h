@@ -98,7 +98,8 @@ struct SmallArgs {      // one-launch path for small problems: complex inputs re
     int identity;
     float alphaR, alphaI, betaR, betaI; int betaZero;
     void* C; long long ldc;
-    int correction;                         // screen + in-kernel FP64 recompute of flagged outputs
+    int correction;                         // screen + in-kernel recompute of flagged outputs:
+                                            // 1 FP64, 2 the reference BLAS's FP32 order
     double corrScale, corrTau;
     unsigned int* flagCount;                // optional: flagged outputs per workgroup (one slot each)
 };
@@ -128,6 +129,8 @@ struct RecomputeArgs {  // FP64 fixed-order recompute of flagged outputs
     const void* shadow;
     void* C; long long ldc;
     unsigned int* counter;
+    int netlib;             // 1: recompute in the reference BLAS's FP32 order instead of FP64
+    float alphaRf, alphaIf, betaRf, betaIf;  // the scalars as given (FP32) for that order
 };
 
 struct ResidualArgs {   // FP64 reference GEMM compared with the FP32 result

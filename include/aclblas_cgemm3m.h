@@ -137,7 +137,9 @@ typedef struct {
     int precision;  /* aclblasCgemm3mPrecision_t */
     int tile;       /* 0 auto, 64 or 128: output tile edge */
     int kSplit;     /* 0 auto, else 1..16 K partitions reduced in fixed order */
-    int correction; /* 0 auto, 1 force on, -1 force off (FP32 only) */
+    int correction; /* 0 auto (on when k >= 1024), 1 on: flagged outputs recomputed in FP64,
+                       2 on: flagged outputs recomputed in the reference BLAS's FP32 order
+                       (Netlib cgemm, bit-identical to it for those outputs), -1 off */
     int identity;   /* 0 auto (fused identity), 1 fused, 2 standard 3M identity, 3 four real
                        products (FP32 only; what aclblasCgemm uses) */
     int variant;    /* 0 auto; 128x128 tiles: 1 four waves through LDS, 2 eight waves through

@@ -313,6 +313,39 @@ def test_correction_fixes_flagged_and_counts(gpu):
     assert abs(complex(on[0]) - z[0]) <= 1e-6 * abs(z[0]) + 1e-30
 
 
+@pytest.mark.parametrize("pin", [dict(), dict(tile=64, k_split=1), dict(tile=128, k_split=2)])
+def test_correction_in_reference_blas_order(gpu, pin):
+    """correction = 2: flagged outputs are recomputed in the reference BLAS's sequential FP32
+    order, so they are bit-identical to the Netlib-order oracle; the rest meet the mixed
+    tolerance.
+    Planner's choice (the one-launch kernel, in-kernel recompute), the packed tile path
+    (recompute kernel) and the K-split path (flags raised by the merge)."""
+    import cgemm3m
+
+    m, n, k = 64, 64, 4096
+    A, B, C = inputs("N", "N", m, n, k, m, k, m, 20261000)
+    Ad = oracle.op_matrix(A, m, k, m, "N")
+    Bd = oracle.op_matrix(B, k, n, k, "N")
+    x = Ad[0, :]
+    v = Bd[:, 0] * 1000.0
+    v = v - (x @ v) / (x @ np.conj(x).T).real * np.conj(x)
+    B2 = B.copy()
+    B2[:k] = v.astype(np.complex64)
+    for beta in (0, BETA2):
+        cfg = cgemm3m.config(correction=2, **pin)
+        st, out = run_gpu(gpu, "N", "N", m, n, k, ALPHA2, A, m, B2, k, beta, C, m, cfg)
+        assert st == 0
+        plan = gpu.last_plan(sync=True)
+        print("plan", plan)
+        assert plan["correction"] == 2 and plan["flagged"] >= 1
+        gold = oracle.cgemm("N", "N", m, n, k, ALPHA2, A, m, B2, k, beta, C, m)
+        assert out[0].tobytes() == gold[0].tobytes(), (out[0], gold[0])
+        # column 0 is scaled by 1e3, so its other outputs are ~1e4-1e5 in size: the mixed
+        # tolerance applies (the verifier's 32-ulp cap is made for the reference's inputs)
+        ok, worst = oracle.reltol_check(region(out, m, n, m), region(gold, m, n, m))
+        assert ok, worst
+
+
 def test_golden_fixtures(gpu):
     """The committed fixtures (tests/golden): every case through the C ABI against its stored
     Netlib-order result (reltol, verifier, padding rows untouched, NaN padding kept)."""
