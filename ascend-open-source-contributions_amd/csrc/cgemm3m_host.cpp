@@ -22,12 +22,12 @@ namespace c3m {
 static const char* kOp = "aclblasCgemm3m";
 // Product-kernel variant used when the caller does not pin one. FP32 128x128 tiles: the
 // LDS-free four-wave kernel (products kernel at 8192^3 on MI355X, complex TFLOP/s: 186 vs
-// 162 for eight waves through LDS); bf16 formats: eight waves per 128x128 tile (bf16x3 390
-// vs 336 for four waves, bf16x6 232 vs 212); FP16 and 64x64 tiles have one form.
+// 162 for eight waves through LDS); 16-bit formats: eight waves per 128x128 tile (bf16x3 390
+// vs 336 for four waves, bf16x6 232 vs 212); 64x64 tiles have one form.
 static int default_variant(int format, int bm)
 {
     if (format == FMT_F32) return bm == 128 ? 2 : 0;
-    return format == FMT_F16 ? 0 : 1;
+    return 1;
 }
 
 StageTimer::StageTimer(aclblasHandle_t h, int stage) : h_(h), stage_(stage)
@@ -183,8 +183,6 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
             tile = tiles_for(128) >= cus ? 128 : 64;
     }
     pl.bm = pl.bn = tile;
-    // The FP16 block-scaled kernel carries two accumulator sets: 128 x 64 or 64 x 64 tiles.
-    if (pl.format == FMT_F16) pl.bn = 64;
     pl.mPad = (int)round_up((size_t)m, 128);
     pl.nPad = (int)round_up((size_t)n, 128);
     pl.kPad = (int)round_up((size_t)k, 32);
@@ -215,15 +213,15 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     }
     pl.path = pl.parts > 1 ? 2 : 1;
     if (cfg && (cfg->variant < 0 || cfg->variant > 6 ||
-                (cfg->variant == 6 && pl.format != FMT_BF16) ||
+                (cfg->variant == 6 && pl.format == FMT_F32) ||
                 (cfg->variant == 5 && (pl.format != FMT_F32 || pl.identity == ID_FOUR || cfg->kSplit > 1)) ||
                 (cfg->variant == 3 && (pl.format != FMT_F32 || (pl.bm != 128 && pl.identity != ID_FOUR))) ||
-                (cfg->variant == 4 && pl.format != FMT_BF16))) {
+                (cfg->variant == 4 && pl.format == FMT_F32))) {
         log_error(op, "invalid variant %d for this precision and tile", cfg->variant);
         return false;
     }
     pl.variant = (cfg && cfg->variant != 0) ? cfg->variant - 1 : default_variant(pl.format, pl.bm);
-    // Separate products (BF16 formats): 256 x 256 tiles of each real product over the whole k
+    // Separate products (16-bit formats): 256 x 256 tiles of each real product over the whole k
     // range, then merge_kernel combines the three FP32 product planes. Chosen by default when
     // nothing is pinned, the fused plan has no K split and the 3 x (256 x 256 tiles) fill at
     // least half the CUs (8192^3 on MI355X, complex TFLOP/s fused vs separate: bf16x3 395 vs
@@ -231,7 +229,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     const bool pinned = cfg && (cfg->tile != 0 || cfg->kSplit != 0 || cfg->variant != 0);
     const long long tiles256 = ((m + 255) / 256) * ((n + 255) / 256);
     const bool products = (cfg && (cfg->variant == 4 || cfg->variant == 6)) ||
-                          (pl.format == FMT_BF16 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2);
+                          (pl.format != FMT_F32 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2);
     if (products) {
         if (cfg && (cfg->kSplit > 1 || (cfg->tile != 0 && cfg->tile != 128))) {
             log_error(op, "the separate-products form has its own tile and no K split");
@@ -244,10 +242,10 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.nPad = (int)round_up((size_t)n, 256);
         // kernel: the register-prefetch form by default; variant 6 pins the LDS-DMA form (byte-
         // identical, measured slower: bf16x3 at 8192^3 7.52 ms against 6.94 ms), whose unrolled
-        // stage loop wants k in whole groups of 4 (bf16x3) or 6 (bf16x6) 16-deep stages
+        // stage loop wants k in whole groups of its unroll depth (16-deep stages)
         pl.variant = (cfg && cfg->variant == 6) ? 5 : 3;
         if (pl.variant == 5) {
-            pl.kPad = (int)round_up((size_t)k, 192);
+            pl.kPad = (int)round_up((size_t)k, 16 * (size_t)lowp_products_lds_unroll(pl.pieces));
             pl.kSteps = pl.kPad / pl.bk;
         }
     }
@@ -321,6 +319,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         const size_t kb = (size_t)(pl.kPad + 127) / 128;
         if (!add_region(3 * (size_t)(pl.mPad / 128) * kb * 4, total, pl.offScaleA)) return false;
         if (!add_region(3 * (size_t)(pl.nPad / 128) * kb * 4, total, pl.offScaleB)) return false;
+        if (!add_region(3 * (size_t)(pl.mPad / 128) * 4, total, pl.offSumA)) return false;
+        if (!add_region(3 * (size_t)(pl.nPad / 128) * 4, total, pl.offSumB)) return false;
     }
     pl.total = total;
     return true;
@@ -386,6 +386,11 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
                              x->identity, const_cast<int*>(x->scaleExp)};
                 if (e == hipSuccess) e = launch_scale_exponents(sa, s);
             }
+            ScaleSummaryArgs sm{const_cast<int*>(pa.scaleExp), 3 * (pl.mPad / 128), const_cast<int*>(pb.scaleExp),
+                                3 * (pl.nPad / 128),
+                                (pl.kPad + 127) / 128, reinterpret_cast<int*>(ws + pl.offSumA),
+                                reinterpret_cast<int*>(ws + pl.offSumB)};
+            if (e == hipSuccess) e = launch_scale_summary(sm, s);
         }
         if (e == hipSuccess) {
             StageTimer t(h, ST_PACK);
@@ -444,6 +449,10 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.scaleA = pa.scaleExp;
         g.scaleB = pb.scaleExp;
         g.kBlocks128 = (pl.kPad + 127) / 128;
+        if (pl.format == FMT_F16) {
+            g.sumA = reinterpret_cast<const int*>(ws + pl.offSumA);
+            g.sumB = reinterpret_cast<const int*>(ws + pl.offSumB);
+        }
         g.correction = pl.correction;
         g.eA = eA;
         g.eB = eB;
@@ -454,13 +463,13 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         if (e == hipSuccess) {
             StageTimer t(h, ST_GEMM);
             if (pl.path == 4)
-                e = launch_gemm_bf16_products(g, pl.pieces, pl.variant == 5, s);
+                e = launch_gemm_lowp_products(g, pl.pieces, pl.variant == 5, s);
             else if (pl.format == FMT_F32)
                 e = launch_gemm_f32(g, pl.bm, pl.bn, pl.variant, s);
             else if (pl.format == FMT_BF16)
                 e = launch_gemm_bf16(g, pl.bm, pl.bn, pl.pieces, pl.variant, s);
             else
-                e = launch_gemm_f16(g, pl.bm, pl.bn, s);
+                e = launch_gemm_f16(g, pl.bm, pl.bn, pl.variant, s);
         }
         if ((pl.parts > 1 || pl.path == 4) && e == hipSuccess) {
             MergeArgs mg{};

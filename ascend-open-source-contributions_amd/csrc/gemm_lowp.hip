@@ -10,9 +10,14 @@
 //  BF16 split (NP = 3, "bf16x6"): x = hi + mid + lo; six MFMAs (hh, hm, mh, hl, lh, mm) keep
 //    every term down to 2^-24 relative (mid*lo, lo*mid, lo*lo dropped).
 //  FP16 block-scaled (F16S): each 128 (mn) x 128 (k) block of every plane was stored as
-//    fp16(x * 2^-e) with its own exponent e; the MFMAs of one 128-deep k block accumulate
-//    into a scratch accumulator that is added to the main one times 2^(eA + eB) (exact
-//    power-of-two scaling), so the block scales can differ along k.
+//    fp16(x * 2^-e) with its own exponent e, so the MFMA products of k block kb carry the
+//    factor 2^(eA + eB). The accumulator holds the running sum in units of the current
+//    block's factor: on entering a block whose factor differs, it is rescaled by the exact
+//    power of two 2^(e_old - e_new) (ldexp), and at the end multiplied by the last factor.
+//    With equal factors along k (the common case) no rescale is issued. Blocks of zeros
+//    (kScaleNone) keep the running factor. Exact while the running sum stays in FP32 range:
+//    a rescale can overflow only if two blocks along k differ by about 2^80 in magnitude,
+//    and a non-finite output is flagged by the correction screen.
 #include "device_util.h"
 #include "launch.h"
 
@@ -21,6 +26,50 @@ namespace c3m {
 // 16-byte lane vector as an ext-vector type: arrays of it stay in VGPRs (an array of the
 // HIP struct uint4 can be moved to LDS by the compiler, serialising the prefetch loads).
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// FP16 separate-products kernels: register sets in flight (register form) and LDS ring depth
+#ifndef C3M_F16_NSET
+#define C3M_F16_NSET 4
+#endif
+#ifndef C3M_F16_LDS_RING
+#define C3M_F16_LDS_RING 6
+#endif
+
+// Factor exponent of one FP16 block product: eA + eB, or kScaleNone if either block is zero.
+__device__ __forceinline__ int block_exp(int ea, int eb)
+{
+    return (ea == kScaleNone || eb == kScaleNone) ? kScaleNone : ea + eb;
+}
+
+// Entering a block with factor exponent e: move the running sum (units 2^cur) to units 2^e.
+template <int TN, int TM>
+__device__ __forceinline__ void enter_block(f32x16 (&acc)[TN][TM], int& cur, int e)
+{
+    if (e == kScaleNone) return;
+    if (cur != kScaleNone && e != cur) {
+        const int d = cur - e;
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[a][b][r] = ldexpf(acc[a][b][r], d);
+    }
+    cur = e;
+}
+
+// End of the k range: running sum times its factor.
+template <int TN, int TM>
+__device__ __forceinline__ void finish_blocks(f32x16 (&acc)[TN][TM], int cur)
+{
+    if (cur == kScaleNone || cur == 0) return;
+#pragma unroll
+    for (int a = 0; a < TN; ++a)
+#pragma unroll
+        for (int b = 0; b < TM; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = ldexpf(acc[a][b][r], cur);
+}
 
 template <int BM, int BN, int NP, int WM = 2, int WN = 2>
 struct LowTile {
@@ -88,7 +137,6 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
     }
 
     f32x16 acc[3][T::TN][T::TM];
-    f32x16 blk[F16S ? 3 : 1][F16S ? T::TN : 1][F16S ? T::TM : 1];
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -96,10 +144,18 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
 #pragma unroll
             for (int b = 0; b < T::TM; ++b)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    acc[q][a][b][r] = 0.0f;
-                    if (F16S) blk[q][a][b][r] = 0.0f;
-                }
+                for (int r = 0; r < 16; ++r) acc[q][a][b][r] = 0.0f;
+    // FP16: running factor exponent per product; the tile lies in one 128 x 128 scale block
+    // of A (row block mb) and of B (nb)
+    int eCur[3] = {kScaleNone, kScaleNone, kScaleNone};
+    const int* scA = nullptr;
+    const int* scB = nullptr;
+    if (F16S) {
+        scA = g.scaleA + (size_t)((tmI * BM) >> 7) * g.kBlocks128;
+        scB = g.scaleB + (size_t)((tnI * BN) >> 7) * g.kBlocks128;
+    }
+    const size_t scStrideA = F16S ? (size_t)(g.mPad >> 7) * g.kBlocks128 : 0;  // per plane
+    const size_t scStrideB = F16S ? (size_t)(g.nPad >> 7) * g.kBlocks128 : 0;
 
     // Same stage pipeline as the FP32 kernel: fragments of stage s to registers, staged
     // stage s+1 into the other LDS buffer, HBM loads of stage s+2, MFMAs, barrier.
@@ -133,6 +189,12 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
         // accumulators between two MFMAs into one accumulator), or one plane at a time when
         // the whole stage would not fit beside the accumulators (three pieces, eight waves).
         constexpr bool PER_PLANE = NP * 3 * (T::TM + T::TN) * 4 > 96;
+        if (F16S && (ks == ks0 || (ks & 7) == 0)) {  // first stage of a 128-deep k block
+            const int kb = ks >> 3;
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                enter_block(acc[q], eCur[q], block_exp(scA[q * scStrideA + kb], scB[q * scStrideB + kb]));
+        }
         if (more) {
 #pragma unroll
             for (int u = 0; u < T::LOADS; ++u)
@@ -161,10 +223,7 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
                     for (int a = 0; a < T::TN; ++a)
 #pragma unroll
                         for (int b = 0; b < T::TM; ++b) {
-                            if (F16S)
-                                blk[q][a][b] = mfma16(bf[0][a], af[0][b], blk[q][a][b], true);
-                            else
-                                acc[q][a][b] = mfma16(bf[TB[t]][a], af[TA[t]][b], acc[q][a][b], false);
+                            acc[q][a][b] = mfma16(bf[TB[t]][a], af[TA[t]][b], acc[q][a][b], F16S);
                         }
             }
         } else {
@@ -188,42 +247,53 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
                     for (int a = 0; a < T::TN; ++a)
 #pragma unroll
                         for (int b = 0; b < T::TM; ++b) {
-                            if (F16S)
-                                blk[q][a][b] = mfma16(bf[q][0][a], af[q][0][b], blk[q][a][b], true);
-                            else
-                                acc[q][a][b] = mfma16(bf[q][TB[t]][a], af[q][TA[t]][b], acc[q][a][b], false);
+                            acc[q][a][b] = mfma16(bf[q][TB[t]][a], af[q][TA[t]][b], acc[q][a][b], F16S);
                         }
-        }
-        if (F16S) {
-            // End of a 128-deep k block (or of this partition): fold the block's products in.
-            const int kb = (ks * 16) >> 7;
-            if (!more || (((ks + 1) * 16) >> 7) != kb) {
-                const int mb = (tmI * BM) >> 7, nb = (tnI * BN) >> 7;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const int e = g.scaleA[((size_t)q * (g.mPad >> 7) + mb) * g.kBlocks128 + kb] +
-                                  g.scaleB[((size_t)q * (g.nPad >> 7) + nb) * g.kBlocks128 + kb];
-#pragma unroll
-                    for (int a = 0; a < T::TN; ++a)
-#pragma unroll
-                        for (int b = 0; b < T::TM; ++b)
-#pragma unroll
-                            for (int r = 0; r < 16; ++r) {
-                                acc[q][a][b][r] += ldexpf(blk[q][a][b][r], e);
-                                blk[q][a][b][r] = 0.0f;
-                            }
-                }
-            }
         }
         __syncthreads();
     }
+    if (F16S)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) finish_blocks(acc[q], eCur[q]);
 
     const int iBase = tmI * BM + wm * (BM / WM) + (lane & 31);
     const int jBase = tnI * BN + wn * (BN / WN) + 4 * (lane >> 5);
     store_tile<T::TN, T::TM, PARTIAL>(g, acc, part, iBase, jBase);
 }
 
-// Separate-products form for the BF16 formats. One workgroup computes one 256 x 256 tile of
+// FP16 separate products: is any of workgroup (q, tmI, tnI)'s two row blocks of A or two
+// column blocks of B mixed (block exponents that differ along k)? Decided per workgroup, so
+// all four waves take the same kernel.
+__device__ __forceinline__ bool f16_tile_mixed(const GemmArgs& g, int q, int tmI, int tnI)
+{
+    const int* sa = g.sumA + q * (g.mPad >> 7) + 2 * tmI;
+    const int* sb = g.sumB + q * (g.nPad >> 7) + 2 * tnI;
+    return sa[0] == kScaleMixed || sa[1] == kScaleMixed || sb[0] == kScaleMixed || sb[1] == kScaleMixed;
+}
+
+// Writes one wave's 128 x (32 TN) outputs of product P_q starting at column block jb of the
+// wave's 128 columns (FP16: times 2^e, e the final factor).
+template <bool F16S, int TN>
+__device__ __forceinline__ void store_products(const GemmArgs& g, f32x16 (&acc)[TN][4], int q, int tmI, int tnI,
+                                               int wm, int wn, int jb, int lane, int e)
+{
+    const int sh = (F16S && e != kScaleNone) ? e : 0;  // ldexp by 0 is exact (identity)
+    float* P = g.partial + (size_t)q * g.partStride;
+    const int iBase = tmI * 256 + wm * 128 + (lane & 31);
+    const int jBase = tnI * 256 + wn * 128 + jb * 32 + 4 * (lane >> 5);
+#pragma unroll
+    for (int a = 0; a < TN; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = iBase + b * 32;
+                const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+                P[(size_t)j * g.mPad + i] = F16S ? ldexpf(acc[a][b][r], sh) : acc[a][b][r];
+            }
+}
+
+// Separate-products form for the 16-bit formats. One workgroup computes one 256 x 256 tile of
 // ONE real product P_q; its 2 x 2 waves each own 128 x 128 outputs (16 accumulators of 32 x 32,
 // all 256 AGPRs). Holding one product per wave instead of three doubles the MFMAs each loaded
 // fragment feeds (4 per term instead of 2), so the operand bytes per MFMA halve and the waves
@@ -232,93 +302,112 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_lowp_kernel(GemmArgs g
 // MFMAs of stage s. Per accumulator the MFMA order is the fused kernel's (stage by stage, the
 // same term order), so each P_q is bit-identical to the fused kernel's product; the FP32
 // planes P_q (ld = mPad) are combined and alpha/beta applied by merge_kernel.
-template <int NP, int NSET>
+//
+// FP16 (F16S) runs as two launches. RESCALE = false takes the workgroups whose block exponents
+// are uniform along k: one factor for the whole k range, applied at the store, and no
+// rescale code in the loop (rescaling 256 AGPR-resident accumulators in the loop makes the
+// compiler spill in every stage). RESCALE = true takes the mixed workgroups and rescales the
+// running sum at block boundaries like the fused kernel; so that the rescale does not make
+// the compiler spill, each wave covers its 128 columns in two passes of 64 (128 accumulator
+// registers), reading its A fragments twice. Per accumulator the MFMA order is the same in
+// every form, and with uniform exponents the rescale never fires, so the split between the
+// two launches does not change any output bit.
+template <int NP, int NSET, bool F16S, bool RESCALE>
 __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_kernel(GemmArgs g)
 {
-    constexpr int TM = 4, TN = 4;
-    constexpr int NTERM = NP == 2 ? 3 : 6;
-    constexpr int TB[6] = {1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
-    constexpr int TA[6] = {NP == 2 ? 0 : 1, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
+    constexpr int TM = 4, TN = RESCALE ? 2 : 4, PASSES = 4 / TN;
+    constexpr int NTERM = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
+    constexpr int TB[6] = {NP == 1 ? 0 : 1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
+    constexpr int TA[6] = {NP == 3 ? 1 : 0, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
     int q, tmI, tnI;
     map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, q, tmI, tnI);
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = wave & 1, wn = wave >> 1;
-    const size_t aRowB = (size_t)(g.mPad / 32) * 1024;  // bytes per 16-deep k fragment row
-    const size_t bRowB = (size_t)(g.nPad / 32) * 1024;
-    const char* aW = reinterpret_cast<const char*>(g.Ap) + (size_t)(q * NP) * g.aPlaneBytes +
-                     (size_t)(tmI * 8 + wm * TM) * 1024;
-    const char* bW = reinterpret_cast<const char*>(g.Bp) + (size_t)(q * NP) * g.bPlaneBytes +
-                     (size_t)(tnI * 8 + wn * TN) * 1024;
-    const unsigned laneB = (unsigned)lane * 16u;
+    if (F16S && f16_tile_mixed(g, q, tmI, tnI) != RESCALE) return;
+    for (int pass = 0; pass < PASSES; ++pass) {
+        const int lane = threadIdx.x & 63;
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int wm = wave & 1, wn = wave >> 1;
+        const size_t aRowB = (size_t)(g.mPad / 32) * 1024;  // bytes per 16-deep k fragment row
+        const size_t bRowB = (size_t)(g.nPad / 32) * 1024;
+        const char* aW = reinterpret_cast<const char*>(g.Ap) + (size_t)(q * NP) * g.aPlaneBytes +
+                         (size_t)(tmI * 8 + wm * TM) * 1024;
+        const char* bW = reinterpret_cast<const char*>(g.Bp) + (size_t)(q * NP) * g.bPlaneBytes +
+                         (size_t)(tnI * 8 + wn * 4 + pass * TN) * 1024;
+        const unsigned laneB = (unsigned)lane * 16u;
 
-    struct Set {
-        u32x4 a[NP][TM], b[NP][TN];
-    };
-    auto load = [&](Set& S, int ks) {
+        struct Set {
+            u32x4 a[NP][TM], b[NP][TN];
+        };
+        auto load = [&](Set& S, int ks) {
 #pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            const char* a = aW + (size_t)p * g.aPlaneBytes + (size_t)ks * aRowB;
-            const char* b = bW + (size_t)p * g.bPlaneBytes + (size_t)ks * bRowB;
+            for (int p = 0; p < NP; ++p) {
+                const char* a = aW + (size_t)p * g.aPlaneBytes + (size_t)ks * aRowB;
+                const char* b = bW + (size_t)p * g.bPlaneBytes + (size_t)ks * bRowB;
 #pragma unroll
-            for (int i = 0; i < TM; ++i) S.a[p][i] = *reinterpret_cast<const u32x4*>(a + i * 1024 + laneB);
+                for (int i = 0; i < TM; ++i) S.a[p][i] = *reinterpret_cast<const u32x4*>(a + i * 1024 + laneB);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) S.b[p][j] = *reinterpret_cast<const u32x4*>(b + j * 1024 + laneB);
-        }
-    };
-    f32x16 acc[TN][TM];
-#pragma unroll
-    for (int a = 0; a < TN; ++a)
-#pragma unroll
-        for (int b = 0; b < TM; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-    auto mfma = [&](const Set& S) {
-#pragma unroll
-        for (int t = 0; t < NTERM; ++t)
-#pragma unroll
-            for (int a = 0; a < TN; ++a)
-#pragma unroll
-                for (int b = 0; b < TM; ++b) acc[a][b] = mfma16(S.b[TB[t]][a], S.a[TA[t]][b], acc[a][b], false);
-    };
-
-    // Ring of NSET register sets: sets 0..NSET-2 preloaded; stage st computes from set
-    // st % NSET after issuing the loads of stage st + NSET - 1 into the set freed last.
-    const int nst = g.kSteps;
-    Set S[NSET];
-#pragma unroll
-    for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nst - 1));
-    int st = 0;
-    for (; st + NSET <= nst; st += NSET) {
-#pragma unroll
-        for (int u = 0; u < NSET; ++u) {
-            // Unconditional (clamped to the last stage): a load skipped on one path would make
-            // the wait before the next MFMAs cover the loads just issued.
-            load(S[(u + NSET - 1) % NSET], min(st + u + NSET - 1, nst - 1));
-            // Keep the loads ahead of the MFMAs: the scheduler would otherwise sink each load
-            // next to its first use and wait on it there.
-            __builtin_amdgcn_sched_barrier(0);
-            mfma(S[u]);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u + 1 < NSET; ++u)
-        if (st + u < nst) mfma(S[u]);
-
-    float* P = g.partial + (size_t)q * g.partStride;
-    const int iBase = tmI * 256 + wm * 128 + (lane & 31);
-    const int jBase = tnI * 256 + wn * 128 + 4 * (lane >> 5);
-#pragma unroll
-    for (int a = 0; a < TN; ++a)
-#pragma unroll
-        for (int b = 0; b < TM; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = iBase + b * 32;
-                const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
-                P[(size_t)j * g.mPad + i] = acc[a][b][r];
+                for (int j = 0; j < TN; ++j) S.b[p][j] = *reinterpret_cast<const u32x4*>(b + j * 1024 + laneB);
             }
+        };
+        f32x16 acc[TN][TM];
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+        // FP16: the wave's 128 x 128 outputs lie in row block mb = 2 tmI + wm of A and nb =
+        // 2 tnI + wn of B. Uniform: one factor exponent from the summaries. Rescale: the running
+        // exponent, with the next block's loaded one block ahead.
+        const int mb = 2 * tmI + wm, nb = 2 * tnI + wn;
+        const int* scA = nullptr;
+        const int* scB = nullptr;
+        int eCur = kScaleNone, eNext = kScaleNone;
+        if (F16S && !RESCALE) eCur = block_exp(g.sumA[q * (g.mPad >> 7) + mb], g.sumB[q * (g.nPad >> 7) + nb]);
+        if (F16S && RESCALE) {
+            scA = g.scaleA + ((size_t)q * (g.mPad >> 7) + mb) * g.kBlocks128;
+            scB = g.scaleB + ((size_t)q * (g.nPad >> 7) + nb) * g.kBlocks128;
+            eNext = block_exp(scA[0], scB[0]);
+        }
+        // MFMAs of stage ks (after the block entry on the first stage of a 128-deep k block)
+        auto mfma = [&](const Set& S, int ks) {
+            if (F16S && RESCALE && (ks & 7) == 0) {
+                enter_block(acc, eCur, eNext);
+                const int kb = (ks >> 3) + 1;
+                if (kb < g.kBlocks128) eNext = block_exp(scA[kb], scB[kb]);
+            }
+#pragma unroll
+            for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+                for (int a = 0; a < TN; ++a)
+#pragma unroll
+                    for (int b = 0; b < TM; ++b) acc[a][b] = mfma16(S.b[TB[t]][a], S.a[TA[t]][b], acc[a][b], F16S);
+        };
+
+        // Ring of NSET register sets: sets 0..NSET-2 preloaded; stage st computes from set
+        // st % NSET after issuing the loads of stage st + NSET - 1 into the set freed last.
+        const int nst = g.kSteps;
+        Set S[NSET];
+#pragma unroll
+        for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nst - 1));
+        int st = 0;
+        for (; st + NSET <= nst; st += NSET) {
+#pragma unroll
+            for (int u = 0; u < NSET; ++u) {
+                // Unconditional (clamped to the last stage): a load skipped on one path would make
+                // the wait before the next MFMAs cover the loads just issued.
+                load(S[(u + NSET - 1) % NSET], min(st + u + NSET - 1, nst - 1));
+                // Keep the loads ahead of the MFMAs: the scheduler would otherwise sink each load
+                // next to its first use and wait on it there.
+                __builtin_amdgcn_sched_barrier(0);
+                mfma(S[u], st + u);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u + 1 < NSET; ++u)
+            if (st + u < nst) mfma(S[u], st + u);
+        store_products<F16S, TN>(g, acc, q, tmI, tnI, wm, wn, pass * TN, lane, eCur);
+    }
 }
 
 // LDS-DMA form of the separate-products kernel. The workgroup's four waves load each stage's
@@ -330,13 +419,15 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_kernel(GemmArgs g
 // issue the loads of stage t+R into stage t's slot, read stage t+1's fragments into the other
 // register set, then the MFMAs of stage t. k is padded to a multiple of 16*U (planner), so the
 // unrolled loop needs no tail. Same MFMA order per accumulator as the other forms.
-template <int NP, int R>
+// FP16: the uniform-exponent workgroups only (the register form's RESCALE launch takes the
+// others; the decision is per workgroup, so no wave leaves a barrier early).
+template <int NP, int R, bool F16S>
 __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmArgs g)
 {
     constexpr int TM = 4, TN = 4;
-    constexpr int NTERM = NP == 2 ? 3 : 6;
-    constexpr int TB[6] = {1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
-    constexpr int TA[6] = {NP == 2 ? 0 : 1, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
+    constexpr int NTERM = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
+    constexpr int TB[6] = {NP == 1 ? 0 : 1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
+    constexpr int TA[6] = {NP == 3 ? 1 : 0, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
     constexpr int SLOT = 16 * NP * 64;                // u32x4 per stage: 8*NP A + 8*NP B fragments
     constexpr int U = (R % 2 == 0) ? R : 2 * R;       // unroll: slot and register set compile-time
     constexpr int PER = 4 * NP;                       // loads per wave per stage
@@ -347,6 +438,7 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmAr
 
     int q, tmI, tnI;
     map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, q, tmI, tnI);
+    if (F16S && f16_tile_mixed(g, q, tmI, tnI)) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave & 1, wn = wave >> 1;
@@ -393,13 +485,15 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmAr
         for (int b = 0; b < TM; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+    const int e = F16S ? block_exp(g.sumA[q * (g.mPad >> 7) + 2 * tmI + wm], g.sumB[q * (g.nPad >> 7) + 2 * tnI + wn])
+                       : kScaleNone;
     auto mfma = [&](const Set& S) {
 #pragma unroll
         for (int t = 0; t < NTERM; ++t)
 #pragma unroll
             for (int a = 0; a < TN; ++a)
 #pragma unroll
-                for (int b = 0; b < TM; ++b) acc[a][b] = mfma16(S.b[TB[t]][a], S.a[TA[t]][b], acc[a][b], false);
+                for (int b = 0; b < TM; ++b) acc[a][b] = mfma16(S.b[TB[t]][a], S.a[TA[t]][b], acc[a][b], F16S);
     };
 
     Set S[2];
@@ -423,42 +517,49 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmAr
     }
     // no load may still be writing LDS when the workgroup ends
     __builtin_amdgcn_s_waitcnt(0);
-
-    float* P = g.partial + (size_t)q * g.partStride;
-    const int iBase = tmI * 256 + wm * 128 + (lane & 31);
-    const int jBase = tnI * 256 + wn * 128 + 4 * (lane >> 5);
-#pragma unroll
-    for (int a = 0; a < TN; ++a)
-#pragma unroll
-        for (int b = 0; b < TM; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = iBase + b * 32;
-                const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
-                P[(size_t)j * g.mPad + i] = acc[a][b][r];
-            }
+    store_products<F16S, TN>(g, acc, q, tmI, tnI, wm, wn, 0, lane, e);
 }
 
 // g.tilesM / g.tilesN count 256 x 256 tiles (mPad, nPad multiples of 256); g.partial holds
 // three FP32 planes of mPad x nPad, g.partStride floats apart; the whole k range (no split).
-hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s)
+// pieces: 2 bf16x3, 3 bf16x6, 1 FP16 block-scaled. The LDS form wants kSteps in whole groups
+// of lowp_products_lds_unroll(pieces) stages.
+int lowp_products_lds_unroll(int pieces)
+{
+    // the kernel unrolls R stages (R even) or 2R; ring depth R = 4 (bf16x3), 3 (bf16x6)
+    const int r = pieces == 2 ? 4 : (pieces == 3 ? 3 : C3M_F16_LDS_RING);
+    return r % 2 == 0 ? r : 2 * r;
+}
+
+hipError_t launch_gemm_lowp_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s)
 {
     const int blocks = 3 * g.tilesM * g.tilesN;
     if (lds) {
+        if (g.kSteps % lowp_products_lds_unroll(pieces)) return hipErrorInvalidValue;
         if (pieces == 2)
-            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<2, 4>), dim3(blocks), dim3(256), 0, s, g);
+            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<2, 4, false>), dim3(blocks), dim3(256), 0, s, g);
         else if (pieces == 3)
-            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<3, 3>), dim3(blocks), dim3(256), 0, s, g);
+            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<3, 3, false>), dim3(blocks), dim3(256), 0, s, g);
+        else if (pieces == 1)
+            hipLaunchKernelGGL((gemm3m_lowp_products_lds_kernel<1, C3M_F16_LDS_RING, true>), dim3(blocks), dim3(256),
+                               0, s, g);
         else
             return hipErrorInvalidValue;
-        return hipGetLastError();
+    } else {
+        if (pieces == 2)
+            hipLaunchKernelGGL((gemm3m_lowp_products_kernel<2, 3, false, false>), dim3(blocks), dim3(256), 0, s, g);
+        else if (pieces == 3)
+            hipLaunchKernelGGL((gemm3m_lowp_products_kernel<3, 2, false, false>), dim3(blocks), dim3(256), 0, s, g);
+        else if (pieces == 1)
+            hipLaunchKernelGGL((gemm3m_lowp_products_kernel<1, C3M_F16_NSET, true, false>), dim3(blocks), dim3(256),
+                               0, s, g);
+        else
+            return hipErrorInvalidValue;
     }
-    if (pieces == 2)
-        hipLaunchKernelGGL((gemm3m_lowp_products_kernel<2, 3>), dim3(blocks), dim3(256), 0, s, g);
-    else if (pieces == 3)
-        hipLaunchKernelGGL((gemm3m_lowp_products_kernel<3, 2>), dim3(blocks), dim3(256), 0, s, g);
-    else
-        return hipErrorInvalidValue;
+    // FP16: the workgroups with mixed block exponents (most launches return at once)
+    if (pieces == 1)
+        hipLaunchKernelGGL((gemm3m_lowp_products_kernel<1, C3M_F16_NSET, true, true>), dim3(blocks), dim3(256), 0, s,
+                           g);
     return hipGetLastError();
 }
 
@@ -494,12 +595,16 @@ hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, int v
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, hipStream_t s)
+// variant 0: 4 waves per 128x128 tile; 1: 8 waves (two per SIMD).
+hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s)
 {
     const bool partial = g.parts > 1;
-    // Two accumulator sets (running sum + current 128-deep k block): at most 128 x 64.
-    if (bm == 128 && bn == 64)
-        return partial ? launch_one<128, 64, 1, true, true>(g, s) : launch_one<128, 64, 1, true, false>(g, s);
+    if (bm == 128 && bn == 128) {
+        if (variant == 1)
+            return partial ? launch_one<128, 128, 1, true, true, 2, 4>(g, s)
+                           : launch_one<128, 128, 1, true, false, 2, 4>(g, s);
+        return partial ? launch_one<128, 128, 1, true, true>(g, s) : launch_one<128, 128, 1, true, false>(g, s);
+    }
     if (bm == 64 && bn == 64)
         return partial ? launch_one<64, 64, 1, true, true>(g, s) : launch_one<64, 64, 1, true, false>(g, s);
     return hipErrorInvalidValue;

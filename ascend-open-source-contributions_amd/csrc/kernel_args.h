@@ -58,6 +58,20 @@ struct ScaleArgs {      // FP16 block-scale pass: power-of-two exponent per 128x
     int* scaleExp;      // [3][mnPad/128][kPad/128]
 };
 
+// Block exponent of an all-zero (or all-padding) FP16 block: its products are exactly zero,
+// so the product kernels leave the running sum's exponent unchanged across it.
+constexpr int kScaleNone = -(1 << 20);
+// Summary of one (plane, 128-row block) along k, after snapping: its exponent if every non-zero
+// block has the same one, kScaleNone if all are zero, kScaleMixed otherwise.
+constexpr int kScaleMixed = -(1 << 20) - 1;
+
+struct ScaleSummaryArgs {  // FP16: snap the block exponents, then summarise each row block
+    int* scaleA; int rowsA;         // rows = 3 * mnPad / 128, kBlocks exponents each
+    int* scaleB; int rowsB;
+    int kBlocks;
+    int* sumA; int* sumB;           // one int per row
+};
+
 struct EnergyArgs {
     const double* partialA; int mnPadA;
     const double* partialB; int mnPadB;
@@ -80,8 +94,9 @@ struct GemmArgs {
     void* C; long long ldc;                 // output (fused path)
     float* partial; size_t partStride;      // K-split: partition p at partial + p*partStride,
                                             // re plane then im plane, each mPad*nPad, ld = mPad
-    // FP16 block scaling
+    // FP16 block scaling: exponents [3][pad/128][kBlocks128] and their per-row-block summaries
     const int* scaleA; const int* scaleB; int kBlocks128;
+    const int* sumA; const int* sumB;
     // rounding correction (fused path only)
     int correction;
     const double* eA; const double* eB;

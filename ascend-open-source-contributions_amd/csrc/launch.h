@@ -9,11 +9,13 @@ namespace c3m {
 hipError_t launch_pack(const PackArgs& a, hipStream_t s);
 hipError_t launch_pack_pair(const PackArgs& a, const PackArgs& b, hipStream_t s);
 hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s);
+hipError_t launch_scale_summary(const ScaleSummaryArgs& a, hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s);
 hipError_t launch_gemm_bf16(const GemmArgs& g, int bm, int bn, int pieces, int variant, hipStream_t s);
-hipError_t launch_gemm_bf16_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s);
-hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, hipStream_t s);
+hipError_t launch_gemm_lowp_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s);
+int lowp_products_lds_unroll(int pieces);
+hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s);
 hipError_t launch_gemm_small(const SmallArgs& g, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
 hipError_t launch_scale_c(const ScaleCArgs& a, hipStream_t s);

@@ -12,6 +12,8 @@
 // format the GEMM kernel copies verbatim. Rows >= mn and k >= kk are written as zeros, so
 // the GEMM never bounds-checks its inner loop. Optional outputs: per-row partial energies
 // sum |x|^2 over the tile's 64 k (FP64, fixed order) for the rounding-correction screen.
+#include <climits>
+
 #include "device_util.h"
 #include "launch.h"
 
@@ -189,7 +191,8 @@ hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blo
 }
 
 // FP16 block scale: exponent e per (plane, 128 mn, 128 k) block with max|x| * 2^-e in
-// [2^14, 2^15), so the block's largest value keeps full FP16 precision without overflow.
+// [2^14, 2^15), so the block's largest value keeps full FP16 precision without overflow;
+// kScaleNone for a block that holds only zeros.
 __global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)
 {
     __shared__ float red[3][256];
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)
     }
     if (threadIdx.x < 3) {
         const float m = red[threadIdx.x][0];
-        int e = 0;
+        int e = m == 0.0f ? kScaleNone : 0;
         if (m > 0.0f && isfinite(m)) {
             int ex;
             frexpf(m, &ex);  // m = f * 2^ex, f in [0.5, 1)
@@ -232,6 +235,63 @@ __global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)
 hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s)
 {
     hipLaunchKernelGGL(scale_exp_kernel, dim3(a.mnPad / 128, (a.kPad + 127) / 128), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+
+// One wave per (plane, 128-row block) of A then of B, after the scale pass and before the pack.
+// Snap: a block whose exponent is within kSnapOctaves of the row block's largest takes the
+// largest, so block maxima that merely straddle a power of two (most data) give one exponent
+// along k; its values stay FP16 normals down to 2^(29 - kSnapOctaves) below its own maximum.
+// Summary: that common exponent, kScaleNone if every block is zero, kScaleMixed if blocks more
+// than kSnapOctaves apart remain.
+constexpr int kSnapOctaves = 8;
+
+__global__ __launch_bounds__(256) void scale_summary_kernel(ScaleSummaryArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool isA = t < a.rowsA;
+    const int row = isA ? t : t - a.rowsA;
+    if (!isA && row >= a.rowsB) return;  // whole wave
+    int* e = (isA ? a.scaleA : a.scaleB) + (size_t)row * a.kBlocks;
+    int top = INT_MIN;
+    for (int kb = lane; kb < a.kBlocks; kb += 64)
+        if (e[kb] != kScaleNone) top = max(top, e[kb]);
+    top = wave_max(top);
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int kb = lane; kb < a.kBlocks; kb += 64) {
+        int x = e[kb];
+        if (x == kScaleNone) continue;
+        if (x != top && x >= top - kSnapOctaves) {
+            x = top;
+            e[kb] = x;
+        }
+        lo = min(lo, x);
+        hi = max(hi, x);
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (lane == 0) (isA ? a.sumA : a.sumB)[row] = hi == INT_MIN ? kScaleNone : (lo == hi ? hi : kScaleMixed);
+}
+
+hipError_t launch_scale_summary(const ScaleSummaryArgs& a, hipStream_t s)
+{
+    const int rows = a.rowsA + a.rowsB;  // one wave each, four per workgroup
+    hipLaunchKernelGGL(scale_summary_kernel, dim3((rows + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

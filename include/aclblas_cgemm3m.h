@@ -145,11 +145,11 @@ typedef struct {
     int variant;    /* 0 auto; 128x128 tiles: 1 four waves through LDS, 2 eight waves through
                        LDS (two per SIMD), 3 four waves loading fragments straight into
                        registers, no LDS (FP32 only); 4 separate products: each real product
-                       on its own 256x256 tiles, then a combine pass (bf16x3/bf16x6 only);
+                       on its own 256x256 tiles, then a combine pass (16-bit precisions);
                        5 one-launch small-problem kernel reading A and B directly, no
                        workspace (FP32 3M, no K split); 6 separate products with the
                        fragments loaded once per workgroup into an LDS ring by direct
-                       global-to-LDS loads (bf16x3/bf16x6; measured slower than 4) */
+                       global-to-LDS loads (16-bit precisions; measured slower than 4) */
 } aclblasCgemm3mConfig;
 
 /* For callers that choose the arithmetic or pin the plan (bench, tests, autotuner, the

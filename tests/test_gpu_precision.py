@@ -66,11 +66,54 @@ def test_precision_vs_fp64(gpu, prec, ops):
 
 @pytest.mark.parametrize("prec", ["bf16x3", "bf16x6", "fp16"])
 def test_precision_ksplit_and_tiles(gpu, prec):
-    for tile, parts in ((64, 1), (128, 1), (64, 3)):
-        if prec == "fp16" and tile == 128:
-            continue
+    for tile, parts in ((64, 1), (128, 1), (64, 3), (128, 2)):
         _, res, _ = _run(gpu, prec, "N", "C", 200, 260, 777, 20261200, tile=tile, parts=parts)
         assert res["normwise"] <= BOUND[prec], res
+
+
+def test_fp16_block_exponents_mixed_and_zero(gpu):
+    """FP16 block exponents that differ along k: columns 128..255 of A scaled by 2^20 in rows
+    0..255 only (so A's first two 128-row blocks have mixed exponents and the rest uniform ones),
+    columns 384..511 zero (an all-zero block, whose exponent is ignored) and columns 640..767
+    scaled by 3 (an exponent one higher, snapped to the row block's). The separate-products
+    form then splits its workgroups between the uniform launch and the rescaling launch; every FP16
+    form without a K split (fused 4 and 8 waves, separate products, its LDS form) gives the same
+    bytes, and (also with a K split) each row band (the mixed rows 0..255, the uniform rest) stays within the
+    FP16 bound against FP64 on its own: a rescale applied wrongly would put the unscaled
+    contributions of rows 0..255 off by 2^20."""
+    import cgemm3m
+
+    m, n, k = 700, 600, 1000
+    print("case fp16 mixed exponents m=%d n=%d k=%d" % (m, n, k))
+    sys.stdout.flush()
+    A, B, C = inputs("N", "N", m, n, k, m, k, m, 20261500)
+    cols = A.reshape(k, m).copy()  # column-major: row l of this view is column l of A
+    cols[128:256, :256] *= np.float32(2.0 ** 20)
+    cols[384:512, :] = 0
+    cols[640:768, :] *= np.float32(3.0)  # one octave up: snapped to the row block's exponent
+    A = cols.reshape(-1)
+    dA, dB = to_dev(A), to_dev(B)
+    outs = {}
+    for variant, parts in ((1, 1), (2, 1), (4, 1), (6, 1), (1, 3)):
+        cfg = cgemm3m.config("fp16", tile=128, k_split=parts, variant=variant, correction=-1)
+        st, out = run_gpu(gpu, "N", "N", m, n, k, ALPHA, A, m, B, k, BETA, C, m, cfg, dA, dB)
+        assert st == 0
+        plan = gpu.last_plan(sync=True)
+        assert plan["path"] == (4 if variant in (4, 6) else (2 if parts > 1 else 1)), plan
+        outs[(variant, parts)] = out
+    first = outs[(1, 1)]
+    differ = {key: int(np.count_nonzero(out.view(np.uint64) != first.view(np.uint64)))
+              for key, out in outs.items() if key[1] == 1}  # a K split sums in another order
+    print("outputs differing from the fused kernel's, per form:", differ)
+    assert not any(differ.values()), differ
+    dC0 = to_dev(C)
+    for key in ((1, 1), (1, 3)):
+        dOut = to_dev(outs[key])
+        for r0, rows in ((0, 256), (256, m - 256)):
+            res = cgemm3m.residual(gpu, "N", "N", rows, n, k, ALPHA, ptr(dA[r0:]), m, ptr(dB), k, BETA,
+                                   ptr(dC0[r0:]), ptr(dOut[r0:]), m)
+            print("form", key, "rows", r0, rows, "err", res)
+            assert res["normwise"] <= BOUND["fp16"], res
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16"])
