@@ -121,10 +121,14 @@ __device__ __forceinline__ void combine3(int identity, float p0, float p1, float
     }
 }
 
-// alpha*(re, im) + beta*c0 in FP32, C never read when beta == 0.
+// alpha*(re, im) + beta*c0 in FP32, C never read when beta == 0. Every product and sum is
+// rounded on its own (no fused multiply-add: the reference BLAS's order for alpha*temp +
+// beta*c), so each kernel that applies the scalars rounds the same way whatever the compiler
+// would otherwise contract.
 __device__ __forceinline__ float2 apply_scalars(float re, float im, float ar, float ai, float br, float bi,
                                                 bool betaZero, float2 c0)
 {
+#pragma clang fp contract(off)
     float2 c;
     c.x = ar * re - ai * im;
     c.y = ar * im + ai * re;
@@ -136,13 +140,15 @@ __device__ __forceinline__ float2 apply_scalars(float re, float im, float ar, fl
 }
 
 // Rounding-correction screen: flag an output whose FP32 error estimate
-// corrScale * |op(A)_i|_2 * |op(B)_j|_2 exceeds corrTau * |c|, or that is not finite.
+// corrScale * |op(A)_i|_2 * |op(B)_j|_2 exceeds corrTau * |c|, or that is not finite. Both
+// sides are non-negative, so their squares are compared (ea, eb are the squared norms): no
+// square root per output; a NaN or infinite estimate still flags.
 __device__ __forceinline__ bool needs_recompute(float2 c, double ea, double eb, double corrScale, double corrTau)
 {
     if (!isfinite(c.x) || !isfinite(c.y)) return true;
-    const double mag = sqrt((double)c.x * c.x + (double)c.y * c.y);
-    const double est = corrScale * sqrt(ea * eb);
-    return !(est <= corrTau * mag);
+    const double mag2 = (double)c.x * c.x + (double)c.y * c.y;
+    const double est2 = (corrScale * corrScale) * (ea * eb);
+    return !(est2 <= (corrTau * corrTau) * mag2);
 }
 
 __device__ __forceinline__ void flag_output(unsigned long long* bitmap, long long lin)

@@ -52,11 +52,60 @@ __device__ __forceinline__ void merge_one(const MergeArgs& a, int i, int j)
     C[co] = c;
 }
 
+// Rows i0..i0+3 of column j (i0 % 4 == 0, ldp % 4 == 0): the same arithmetic per output as
+// merge_one, with the partial or product planes read as 16-byte vectors.
+__device__ __forceinline__ void merge_four(const MergeArgs& a, int i0, int j)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const size_t o = (size_t)j * a.ldp + i0;
+    const size_t plane = a.partStride;
+    const size_t imOff = plane / 2;
+    const f4* P = reinterpret_cast<const f4*>(a.partial);
+    f4 re = {0.0f, 0.0f, 0.0f, 0.0f}, im = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (a.products) {
+        const f4 p0 = P[o / 4], p1 = P[(plane + o) / 4], p2 = P[(2 * plane + o) / 4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float r, q;
+            combine3(a.identity, p0[u], p1[u], p2[u], r, q);
+            re[u] = r;
+            im[u] = q;
+        }
+    } else {
+        for (int p = 0; p < a.parts; ++p) {
+            const f4 pr = P[((size_t)p * plane + o) / 4], pi = P[((size_t)p * plane + imOff + o) / 4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                re[u] += pr[u];
+                im[u] += pi[u];
+            }
+        }
+    }
+    float2* C = reinterpret_cast<float2*>(a.C);
+    const size_t co = (size_t)j * a.ldc + i0;
+    const bool bz = a.betaZero != 0;
+    const int live = a.m - i0 < 4 ? a.m - i0 : 4;
+    float2 c0[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c0[u] = (!bz && u < live) ? C[co + u] : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (u >= live) break;
+        const float2 c = apply_scalars(re[u], im[u], a.alphaR, a.alphaI, a.betaR, a.betaI, bz, c0[u]);
+        if (a.correction && needs_recompute(c, a.eA[i0 + u], a.eB[j], a.corrScale, a.corrTau)) {
+            const long long lin = (long long)j * a.m + i0 + u;
+            flag_output(a.bitmap, lin);
+            if (!bz) reinterpret_cast<float2*>(a.shadow)[lin] = c0[u];
+        }
+        C[co + u] = c;
+    }
+}
+
 __global__ __launch_bounds__(256) void merge_kernel(MergeArgs a)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.m) return;
-    for (int j = blockIdx.y; j < a.n; j += gridDim.y) merge_one(a, i, j);
+    const int i0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i0 >= a.m) return;
+    for (int j = blockIdx.y; j < a.n; j += gridDim.y) merge_four(a, i0, j);
 }
 
 // Grouped batch: blockIdx.y picks the K-split problem, x walks its m*n outputs.
@@ -81,7 +130,8 @@ static unsigned grid_y(int n) { return (unsigned)(n < 65535 ? n : 65535); }
 
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(merge_kernel, dim3((a.m + 255) / 256, grid_y(a.n)), dim3(256), 0, s, a);
+    if (a.ldp % 4) return hipErrorInvalidValue;  // planes are padded to 128 rows
+    hipLaunchKernelGGL(merge_kernel, dim3((a.m + 1023) / 1024, grid_y(a.n)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -130,53 +180,78 @@ __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
     const float2* A = reinterpret_cast<const float2*>(a.A);
     const float2* B = reinterpret_cast<const float2*>(a.B);
     float2* C = reinterpret_cast<float2*>(a.C);
-    if (a.netlib) {
-        // one flagged output per lane, each in the reference BLAS's sequential FP32 order
-        for (long long w = wave; w < a.words; w += waves) {
-            const unsigned long long bits = a.bitmap[w];
-            if ((bits >> lane) & 1ull) {
-                const long long lin = w * 64 + lane;
-                const long long i = lin % a.m, j = lin / a.m;
-                const float2 c0 = a.betaZero ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.shadow)[lin];
-                C[(size_t)j * a.ldc + i] =
-                    netlib_output(A, a.lda, a.opA, B, a.ldb, a.opB, i, j, a.k, make_float2(a.alphaRf, a.alphaIf),
-                                  make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
-                if (a.counter != nullptr) atomicAdd(a.counter, 1u);
-            }
-        }
-        return;
-    }
-    for (long long w = wave; w < a.words; w += waves) {
-        unsigned long long bits = a.bitmap[w];
-        while (bits != 0ull) {
-            const int b = __ffsll((long long)bits) - 1;
-            bits &= bits - 1ull;
-            const long long lin = w * 64 + b;
-            const long long i = lin % a.m, j = lin / a.m;
-            double sr = 0.0, si = 0.0;
-            for (long long l = lane; l < a.k; l += 64) {
-                const double2 x = load_op(A, a.lda, a.opA, i, l, false);   // op(A)(i, l)
-                const double2 y = load_op(B, a.ldb, a.opB, l, j, false);   // op(B)(l, j)
-                sr = fma(x.x, y.x, sr);
-                sr = fma(-x.y, y.y, sr);
-                si = fma(x.x, y.y, si);
-                si = fma(x.y, y.x, si);
-            }
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                sr += __shfl_xor(sr, d, 64);
-                si += __shfl_xor(si, d, 64);
-            }
-            if (lane == 0) {
-                double cr = a.alphaR * sr - a.alphaI * si;
-                double ci = a.alphaR * si + a.alphaI * sr;
-                if (!a.betaZero) {
-                    const float2 c0 = reinterpret_cast<const float2*>(a.shadow)[lin];
-                    cr += a.betaR * c0.x - a.betaI * c0.y;
-                    ci += a.betaR * c0.y + a.betaI * c0.x;
+    // Each wave scans 64 consecutive bitmap words at a time (one per lane) and handles the
+    // nonzero ones in increasing order.
+    for (long long w0 = wave * 64; w0 < a.words; w0 += waves * 64) {
+        const unsigned long long mine = w0 + lane < a.words ? a.bitmap[w0 + lane] : 0ull;
+        unsigned long long live = __ballot(mine != 0ull);
+        while (live != 0ull) {
+            const int src = __ffsll((long long)live) - 1;
+            live &= live - 1ull;
+            const long long w = w0 + src;
+            unsigned long long bits = __shfl(mine, src, 64);
+            if (a.netlib) {
+                // one flagged output per lane, each in the reference BLAS's sequential FP32 order
+                if ((bits >> lane) & 1ull) {
+                    const long long lin = w * 64 + lane;
+                    const long long i = lin % a.m, j = lin / a.m;
+                    const float2 c0 =
+                        a.betaZero ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.shadow)[lin];
+                    C[(size_t)j * a.ldc + i] = netlib_output(A, a.lda, a.opA, B, a.ldb, a.opB, i, j, a.k,
+                                                             make_float2(a.alphaRf, a.alphaIf),
+                                                             make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
+                    if (a.counter != nullptr) atomicAdd(a.counter, 1u);
                 }
-                C[(size_t)j * a.ldc + i] = make_float2((float)cr, (float)ci);
-                if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+                continue;
+            }
+            while (bits != 0ull) {
+                const int b = __ffsll((long long)bits) - 1;
+                bits &= bits - 1ull;
+                const long long lin = w * 64 + b;
+                const long long i = lin % a.m, j = lin / a.m;
+                // lane sums k = lane + 64 t in increasing t; the loads of four t are in flight
+                // together, the adds stay in t order
+                double sr = 0.0, si = 0.0;
+                long long l = lane;
+                for (; l + 192 < a.k; l += 256) {
+                    double2 x[4], y[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        x[u] = load_op(A, a.lda, a.opA, i, l + 64 * u, false);   // op(A)(i, l)
+                        y[u] = load_op(B, a.ldb, a.opB, l + 64 * u, j, false);   // op(B)(l, j)
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        sr = fma(x[u].x, y[u].x, sr);
+                        sr = fma(-x[u].y, y[u].y, sr);
+                        si = fma(x[u].x, y[u].y, si);
+                        si = fma(x[u].y, y[u].x, si);
+                    }
+                }
+                for (; l < a.k; l += 64) {
+                    const double2 x = load_op(A, a.lda, a.opA, i, l, false);
+                    const double2 y = load_op(B, a.ldb, a.opB, l, j, false);
+                    sr = fma(x.x, y.x, sr);
+                    sr = fma(-x.y, y.y, sr);
+                    si = fma(x.x, y.y, si);
+                    si = fma(x.y, y.x, si);
+                }
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) {
+                    sr += __shfl_xor(sr, d, 64);
+                    si += __shfl_xor(si, d, 64);
+                }
+                if (lane == 0) {
+                    double cr = a.alphaR * sr - a.alphaI * si;
+                    double ci = a.alphaR * si + a.alphaI * sr;
+                    if (!a.betaZero) {
+                        const float2 c0 = reinterpret_cast<const float2*>(a.shadow)[lin];
+                        cr += a.betaR * c0.x - a.betaI * c0.y;
+                        ci += a.betaR * c0.y + a.betaI * c0.x;
+                    }
+                    C[(size_t)j * a.ldc + i] = make_float2((float)cr, (float)ci);
+                    if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+                }
             }
         }
     }
@@ -184,7 +259,7 @@ __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)
 {
-    long long blocks = (a.words + 3) / 4;  // 4 waves per block, one bitmap word per wave
+    long long blocks = (a.words + 255) / 256;  // 4 waves per block, 64 bitmap words per wave
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);

@@ -297,20 +297,28 @@ hipError_t launch_scale_summary(const ScaleSummaryArgs& a, hipStream_t s)
 
 // Row energies e[r] = sum over k tiles (index order) of the pack kernel's partials; also
 // clears the correction bitmap and the flagged-output counter for this call.
+// sum over kt of p[kt * ld] in index order; eight loads issued ahead of their adds
+__device__ __forceinline__ double ordered_sum(const double* p, size_t ld, int n)
+{
+    double s = 0.0;
+    int kt = 0;
+    for (; kt + 8 <= n; kt += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(kt + u) * ld];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; kt < n; ++kt) s += p[(size_t)kt * ld];
+    return s;
+}
+
 __global__ __launch_bounds__(256) void energy_kernel(EnergyArgs a)
 {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long stride = (long long)gridDim.x * 256;
-    for (long long r = t; r < a.mnPadA; r += stride) {
-        double s = 0.0;
-        for (int kt = 0; kt < a.kTiles; ++kt) s += a.partialA[(size_t)kt * a.mnPadA + r];
-        a.eA[r] = s;
-    }
-    for (long long r = t; r < a.mnPadB; r += stride) {
-        double s = 0.0;
-        for (int kt = 0; kt < a.kTiles; ++kt) s += a.partialB[(size_t)kt * a.mnPadB + r];
-        a.eB[r] = s;
-    }
+    for (long long r = t; r < a.mnPadA; r += stride) a.eA[r] = ordered_sum(a.partialA + r, a.mnPadA, a.kTiles);
+    for (long long r = t; r < a.mnPadB; r += stride) a.eB[r] = ordered_sum(a.partialB + r, a.mnPadB, a.kTiles);
     for (long long w = t; w < a.bitmapWords; w += stride) a.bitmap[w] = 0ull;
     if (t == 0 && a.counter != nullptr) *a.counter = 0u;
 }
