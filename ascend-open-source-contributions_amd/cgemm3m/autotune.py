@@ -1,21 +1,37 @@
-"""Autotuner for the Cgemm3m planner: measures output tile (64/128) x K partitions per shape
-class on the GPU and writes csrc/tuned_table.h (compiled into the library).
+"""Autotuner for the Cgemm3m planner: measures plan candidates per shape class on the GPU
+and writes csrc/tuned_table.h (compiled into the library).
 
-One-problem classes are timed with aclblasCgemm3mEx and pinned plans; grouped classes are
-timed as one aclblasCgemm3mGroupedBatchedEx call of identical problems with the plan pinned. A class is the
-bin (floor(log2(m*n)), floor(log2(k))). Each candidate's time is the median of `reps`
-event-timed calls after one warm-up. The table is deterministic input to the planner: the
-same arguments always get the same plan.
+A class is the bin (floor(log2(m*n)), floor(log2(k))). One-problem candidates: output tile 64
+or 128 with 1-16 K partitions, and the one-launch small-problem kernel ("tile 32") with 1-16 k
+slices per output tile. Grouped classes are timed as one aclblasCgemm3mGroupedBatchedEx call
+of identical problems with the member plan pinned. The table is deterministic input to the
+planner: the same arguments always get the same plan.
 
-Usage (GPU):  python -m cgemm3m.autotune [--quick] [--write]
+One-problem timing, as the reference measures its performance cases (ops-blas/test/gemm/
+cgemm3m/README.md:95-115): the sum of a call's kernels' device durations from a rocprofv3
+kernel trace, launch gaps excluded. Two steps on the GPU:
+
+  rocprofv3 --kernel-trace --output-format csv -d OUT -o run -- python3 -m cgemm3m.autotune run MANIFEST
+  python3 -m cgemm3m.autotune pick OUT/run_kernel_trace.csv MANIFEST [--write]
+
+`run` separates candidates with one non-library kernel (a torch fill) and writes MANIFEST
+(shapes and candidate order); `pick` chooses, per class, the candidate with the smallest
+geometric mean of time / (that shape's best time) over the class's shapes, and also times the
+grouped classes with events. Without a profiler, `python -m cgemm3m.autotune events` times
+every candidate with events around the call (launch gaps included).
+
+The shapes: a few squares and tall/flat problems, plus the reference's 200 performance-suite
+cases (tests/golden/pf_cases.csv, their ops and leading dimensions).
 """
 import argparse
+import json
 import math
 import os
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.normpath(os.path.join(HERE, "..", "csrc", "tuned_table.h"))
+CALLS = 6  # per candidate; the first two are warm-up
 
 SINGLE_SHAPES = [
     (256, 256, 256), (512, 512, 512), (1024, 1024, 1024), (2048, 2048, 2048), (4096, 4096, 4096),
@@ -29,6 +45,37 @@ GROUPED_SHAPES = [(m, m, k) for m in (32, 64, 128, 256, 512) for k in (64, 256, 
 
 def bin_of(m, n, k):
     return int(math.floor(math.log2(m * n))), int(math.floor(math.log2(k)))
+
+
+def candidates(m, n, k):
+    """(tile, parts) plans worth timing for one problem (tile 32: small kernel, parts = slices)."""
+    steps = (k + 31) // 32 * 2          # 16-deep stages of the packed kernels
+    chunks = (k + 15) // 16             # 16-deep k chunks of the small kernel
+    out = [(tile, parts) for tile in (64, 128) for parts in (1, 2, 4, 8, 16)
+           if parts == 1 or steps // parts >= 2]
+    if m * n <= 2048 * 2048:
+        out += [(32, s) for s in (1, 2, 4, 8, 16) if s == 1 or chunks // s >= 4]
+    return out
+
+
+def _config(cgemm3m, tile, parts):
+    if tile == 32:
+        return cgemm3m.config("fp32", variant=5, k_split=parts)
+    return cgemm3m.config("fp32", tile=tile, k_split=parts)
+
+
+def shape_list(which):
+    """[(ta, tb, m, n, k, lda, ldb, ldc)] for 'builtin', 'pf' or 'all'."""
+    out = []
+    if which in ("builtin", "all"):
+        out += [("N", "N", m, n, k, m, k, m) for (m, n, k) in SINGLE_SHAPES]
+    if which in ("pf", "all"):
+        sys.path.insert(0, os.path.dirname(HERE))
+        from cgemm3m import pf_suite
+
+        out += [(c["transa"], c["transb"], c["m"], c["n"], c["k"], c["lda"], c["ldb"], c["ldc"])
+                for c in pf_suite.load_cases()]
+    return out
 
 
 def choose(res, margin=0.02):
@@ -57,33 +104,112 @@ def _timer(torch):
     return timed
 
 
-def tune_single(h, torch, shapes, reps):
+def _operands(torch, ta, tb, m, n, k, lda, ldb, ldc):
+    A = (torch.rand(2 * lda * (k if ta == "N" else m), device="cuda") * 2 - 1).view(torch.complex64)
+    B = (torch.rand(2 * ldb * (n if tb == "N" else k), device="cuda") * 2 - 1).view(torch.complex64)
+    C = torch.zeros(ldc * n, dtype=torch.complex64, device="cuda")
+    return A, B, C
+
+
+def tune_single_events(h, torch, shapes, reps):
+    """Every candidate timed with events around the call (median of `reps`)."""
     import cgemm3m
 
     timed = _timer(torch)
     out = []
-    for (m, n, k) in shapes:
-        A = torch.rand(2 * m * k, device="cuda").view(torch.complex64)
-        B = torch.rand(2 * k * n, device="cuda").view(torch.complex64)
-        C = torch.zeros(m * n, dtype=torch.complex64, device="cuda")
-        P = cgemm3m.torch_ptr
+    P = cgemm3m.torch_ptr
+    for (ta, tb, m, n, k, lda, ldb, ldc) in shapes:
+        A, B, C = _operands(torch, ta, tb, m, n, k, lda, ldb, ldc)
         res = {}
-        cands = [(tile, parts) for tile in (64, 128) for parts in (1, 2, 4, 8, 16) if parts == 1 or k // parts >= 256]
-        if m * n <= 2048 * 2048:
-            cands.append((32, 1))  # the one-launch small-problem kernel
-        for tile, parts in cands:
-            if tile == 32:
-                cfg = cgemm3m.config("fp32", variant=5)
-            else:
-                cfg = cgemm3m.config("fp32", tile=tile, k_split=parts)
-            res[(tile, parts)] = timed(lambda: cgemm3m.cgemm3m(h, "N", "N", m, n, k, 1.0, P(A), m, P(B), k,
-                                                               0.0, P(C), m, cfg), reps)
-        best = choose(res)
-        print("single %5d x %5d x %6d  best tile %d parts %2d  %.3f ms  (all: %s)" % (
-            m, n, k, best[0], best[1], res[best], {"%d/%d" % kk: round(v, 3) for kk, v in res.items()}),
-            flush=True)
-        out.append((bin_of(m, n, k), 0, best))
+        for tile, parts in candidates(m, n, k):
+            cfg = _config(cgemm3m, tile, parts)
+            res[(tile, parts)] = timed(lambda: cgemm3m.cgemm3m(h, ta, tb, m, n, k, 1.0, P(A), lda, P(B), ldb, 0.0,
+                                                               P(C), ldc, cfg), reps)
+        out.append(((ta, tb, m, n, k), res))
+        print("single %s%s %5d x %5d x %6d  best %s" % (ta, tb, m, n, k, min(res, key=res.get)), flush=True)
         del A, B, C
+    return out
+
+
+def run_trace(h, torch, shapes, manifest):
+    """Every candidate CALLS times, candidates separated by a torch fill (for `pick`)."""
+    import cgemm3m
+
+    P = cgemm3m.torch_ptr
+    marker = torch.zeros(64, device="cuda")
+    entries = []
+    for idx, (ta, tb, m, n, k, lda, ldb, ldc) in enumerate(shapes):
+        A, B, C = _operands(torch, ta, tb, m, n, k, lda, ldb, ldc)
+        cands = candidates(m, n, k)
+        for tile, parts in cands:
+            cfg = _config(cgemm3m, tile, parts)
+            torch.cuda.synchronize()
+            marker.fill_(float(idx))
+            for _ in range(CALLS):
+                cgemm3m.cgemm3m(h, ta, tb, m, n, k, 1.0, P(A), lda, P(B), ldb, 0.0, P(C), ldc, cfg)
+                torch.cuda.synchronize()
+        entries.append({"shape": [ta, tb, m, n, k], "cands": [list(c) for c in cands]})
+        if idx % 20 == 19:
+            print("autotune run: %d / %d shapes" % (idx + 1, len(shapes)), flush=True)
+        del A, B, C
+    torch.cuda.synchronize()
+    marker.fill_(-1.0)
+    torch.cuda.synchronize()
+    with open(manifest, "w") as f:
+        json.dump({"calls": CALLS, "entries": entries}, f)
+
+
+def read_trace(trace, manifest):
+    """[((ta, tb, m, n, k), {(tile, parts): mean kernel-time sum per call in ms})] from `run`."""
+    import csv
+
+    with open(manifest) as f:
+        man = json.load(f)
+    calls = man["calls"]
+    groups, cur = [], None
+    with open(trace) as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Start_Timestamp"]))
+    for r in rows:
+        if "c3m::" in r["Kernel_Name"]:
+            if cur is None:
+                cur = []
+                groups.append(cur)
+            cur.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+        else:
+            cur = None
+    want = sum(len(e["cands"]) for e in man["entries"])
+    if len(groups) != want:
+        raise SystemExit("trace holds %d candidate groups, the manifest %d" % (len(groups), want))
+    out, g = [], 0
+    for e in man["entries"]:
+        res = {}
+        for c in e["cands"]:
+            ks = groups[g]
+            g += 1
+            if len(ks) % calls:
+                raise SystemExit("shape %s candidate %s: %d kernels for %d calls" % (e["shape"], c, len(ks), calls))
+            per = len(ks) // calls
+            times = [sum(ks[i * per:(i + 1) * per]) for i in range(2, calls)]
+            res[tuple(c)] = sum(times) / len(times)
+        out.append((tuple(e["shape"]), res))
+    return out
+
+
+def pick_bins(measured):
+    """Per bin, the candidate timed on every shape of the bin with the smallest geometric mean
+    of time / best time for that shape (ties: fewer partitions, then the larger tile)."""
+    bins = {}
+    for shape, res in measured:
+        bins.setdefault(bin_of(*shape[2:5]), []).append(res)
+    out = []
+    for b, members in sorted(bins.items()):
+        common = set(members[0])
+        for res in members[1:]:
+            common &= set(res)
+        score = {}
+        for c in common:
+            score[c] = math.exp(sum(math.log(res[c] / min(res.values())) for res in members) / len(members))
+        out.append((b, 0, choose(score)))
     return out
 
 
@@ -148,20 +274,39 @@ def write_table(entries, path=TABLE):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=["run", "pick", "events"])
+    ap.add_argument("paths", nargs="*", help="run: MANIFEST; pick: TRACE MANIFEST")
+    ap.add_argument("--shapes", default="all", choices=["builtin", "pf", "all"])
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--write", action="store_true")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=TABLE)
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
+    if args.mode == "pick" and len(args.paths) != 2:
+        raise SystemExit("pick needs TRACE MANIFEST")
+    if args.mode == "run" and len(args.paths) != 1:
+        raise SystemExit("run needs MANIFEST")
     import torch
 
     import cgemm3m
 
     h = cgemm3m.Handle()
-    single = SINGLE_SHAPES[:6] if args.quick else SINGLE_SHAPES
+    shapes = shape_list(args.shapes)
+    if args.quick:
+        shapes = shapes[::10]
+    if args.mode == "run":
+        run_trace(h, torch, shapes, args.paths[0])
+        h.close()
+        return
+    if args.mode == "pick":
+        single = pick_bins(read_trace(args.paths[0], args.paths[1]))
+    else:
+        single = pick_bins(tune_single_events(h, torch, shapes, args.reps))
+    for (b, _, best) in single:
+        print("bin mn 2^%d k 2^%d: tile %d parts %d" % (b[0], b[1], best[0], best[1]))
     grouped = GROUPED_SHAPES[::3] if args.quick else GROUPED_SHAPES
-    entries = tune_single(h, torch, single, args.reps) + tune_grouped(h, torch, grouped, args.reps)
+    entries = single + tune_grouped(h, torch, grouped, args.reps)
     if args.write:
         print("wrote", write_table(entries, args.out))
     h.close()

@@ -214,7 +214,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     pl.path = pl.parts > 1 ? 2 : 1;
     if (cfg && (cfg->variant < 0 || cfg->variant > 6 ||
                 (cfg->variant == 6 && pl.format == FMT_F32) ||
-                (cfg->variant == 5 && (pl.format != FMT_F32 || pl.identity == ID_FOUR || cfg->kSplit > 1)) ||
+                (cfg->variant == 5 && (pl.format != FMT_F32 || pl.identity == ID_FOUR)) ||
                 (cfg->variant == 3 && (pl.format != FMT_F32 || (pl.bm != 128 && pl.identity != ID_FOUR))) ||
                 (cfg->variant == 4 && pl.format == FMT_F32))) {
         log_error(op, "invalid variant %d for this precision and tile", cfg->variant);
@@ -258,28 +258,51 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.correction = cfg->correction > 0 ? cfg->correction : 0;
     else
         pl.correction = k >= 1024;
-    // One-launch small-problem kernel: no pack, no workspace, no correction. Pinned with
-    // variant 5, or chosen when nothing is pinned and the correction is off, if the tuned
-    // table names it (tile 32) for this shape class or, for classes it does not list, the
-    // output has at most 256 of its 16x16 tiles.
+    // One-launch small-problem kernel: no pack launch. Pinned with variant 5 (kSplit = k
+    // slices per tile), or chosen when nothing is pinned if the tuned table names it (tile 32,
+    // parts = slices) for this shape class or, for classes it does not list, the output has
+    // at most 256 of its 16x16 tiles. Workspace: none, unless the correction is on (one
+    // flagged-output counter per tile) or k is sliced (arrival counters, partial products and
+    // energies per slice).
     {
         int tt = 0, tp = 0;
         const bool listed = tuned_lookup(m, n, k, false, &tt, &tp);
-        const bool fits = listed ? tt == 32 : ((m + 15) / 16) * ((n + 15) / 16) <= 256;
+        const long long tiles16 = ((m + 15) / 16) * ((n + 15) / 16);
+        const bool fits = listed ? tt == 32 : tiles16 <= 256;
         const bool autoSmall = pl.format == FMT_F32 && !pinned && pl.identity != ID_FOUR && fits;
         if ((cfg && cfg->variant == 5) || autoSmall) {
             const int correction = pl.correction;
+            const long long chunks = (k + 15) / 16;
+            long long slices = 1;
+            if (cfg && cfg->variant == 5 && cfg->kSplit > 0)
+                slices = cfg->kSplit;
+            else if (listed && tp > 0)
+                slices = tp;
+            else  // about two workgroups per CU, each slice at least 8 chunks (two per wave)
+                slices = std::min<long long>({(2LL * cus + tiles16 - 1) / tiles16, chunks / 8, 16});
+            slices = std::max<long long>(1, std::min(slices, chunks));
             pl = Plan{};
             pl.precision = ACLBLAS_CGEMM3M_FP32;
             pl.identity = (cfg && cfg->identity == ID_STANDARD) ? ID_STANDARD : ID_FUSED;
             pl.path = 5;
             pl.bm = pl.bn = 16;
-            pl.parts = 1;
+            pl.parts = (int)slices;
             pl.variant = 4;
             pl.correction = correction;
-            pl.smallTiles = ((m + 15) / 16) * ((n + 15) / 16);
-            // with the correction on: one flagged-output counter per workgroup
-            pl.total = correction ? (size_t)pl.smallTiles * sizeof(unsigned int) : 0;
+            pl.smallTiles = tiles16;
+            size_t total = 0, bytes = 0, n2 = 0;
+            if (slices > 1) {
+                // whole 8-byte words: the counters are cleared as 64-bit words
+                if (!mul_ok((size_t)(tiles16 + 1) / 2, 8, bytes) || !add_region(bytes, total, pl.offArrive))
+                    return false;
+                if (!mul_ok((size_t)tiles16, (size_t)slices, n2) || !mul_ok(n2, 768 * 4, bytes) ||
+                    !add_region(bytes, total, pl.offPartial))
+                    return false;
+                if (correction && (!mul_ok(n2, 32 * 8, bytes) || !add_region(bytes, total, pl.offEPA))) return false;
+            }
+            if (correction && (!mul_ok((size_t)tiles16, 4, bytes) || !add_region(bytes, total, pl.offCounter)))
+                return false;
+            pl.total = total;
             return true;
         }
     }
@@ -362,12 +385,18 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
     if (pl.path == 5) {
         const double amag =
             std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
-        unsigned int* slots = pl.correction ? reinterpret_cast<unsigned int*>(ws) : nullptr;
+        unsigned int* slots = pl.correction ? reinterpret_cast<unsigned int*>(ws + pl.offCounter) : nullptr;
+        const bool sliced = pl.parts > 1;
         SmallArgs a{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.m, p.n, p.k, pl.identity,
                     p.alpha.real, p.alpha.imag, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc,
-                    pl.correction, amag * 0x1p-24, 0x1p-8, slots};
+                    pl.correction, amag * 0x1p-24, 0x1p-8, slots, pl.parts,
+                    sliced ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr,
+                    sliced && pl.correction ? reinterpret_cast<double*>(ws + pl.offEPA) : nullptr,
+                    sliced ? reinterpret_cast<unsigned int*>(ws + pl.offArrive) : nullptr};
         StageTimer t(h, ST_GEMM);
-        e = launch_gemm_small(a, s);
+        if (sliced)  // the arrival counters start at zero (the kernel leaves them zero)
+            e = launch_fill_u64(reinterpret_cast<unsigned long long*>(ws + pl.offArrive), (pl.smallTiles + 1) / 2, s);
+        if (e == hipSuccess) e = launch_gemm_small(a, s);
         if (slots != nullptr) {
             h->lastCounter = slots;
             h->lastCounterSlots = pl.smallTiles;

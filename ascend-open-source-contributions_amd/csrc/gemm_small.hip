@@ -18,6 +18,12 @@
 // (i0 + l%16, j0 + 4*(l/16) + r) in register r. Out-of-range elements read a clamped
 // in-range address and are replaced by zero, so no load is conditional.
 //
+// K split in the same launch (slices > 1): each tile gets `slices` workgroups, slice s taking
+// the k chunks [chunks*s/slices, chunks*(s+1)/slices); each writes its partial products (and
+// energies) to the workspace, then counts itself in on the tile's arrival counter. The one
+// that arrives last adds the slices' partials in slice order (whichever arrived last, so the
+// result is deterministic), resets the counter to zero and runs the epilogue below.
+//
 // Rounding correction in the same launch: the loads that feed the MFMAs also give each row's
 // |op(A)_i|^2 and each column's |op(B)_j|^2 (FP64, summed in a fixed order); an output that
 // fails the screen (as in the other paths) is recomputed by the whole workgroup in FP64 from
@@ -39,7 +45,7 @@ struct SmallSet {
 
 }  // namespace
 
-__global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int tilesM)
+__global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int tilesM, int tiles)
 {
     __shared__ float red[4][3][256];
     __shared__ double ered[4][2][16];  // per wave: row energies of op(A), column energies of op(B)
@@ -47,9 +53,11 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     __shared__ int flagged;
     __shared__ int flagList[256];
     __shared__ float2 flagC0[256];
+    __shared__ int lastIn;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tm = blockIdx.x % tilesM, tn = blockIdx.x / tilesM;
+    const int tile = blockIdx.x % tiles, slice = blockIdx.x / tiles;
+    const int tm = tile % tilesM, tn = tile / tilesM;
     const int i0 = tm * 16, j0 = tn * 16;
     const int i = i0 + (lane & 15), j = j0 + (lane & 15), t = lane >> 4;
     const bool iIn = i < g.m, jIn = j < g.n;
@@ -64,7 +72,10 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     const float aSign = g.opA == 113 ? -1.0f : 1.0f, bSign = g.opB == 113 ? -1.0f : 1.0f;
     const int K = g.k;
     const int chunks = (K + 15) / 16;
-    const int iters = (chunks + 3) / 4;  // chunks of this wave: wave, wave + 4, ...
+    // this slice's chunks [cs0, cs1) (at least one: the planner keeps slices <= chunks)
+    const int cs0 = (int)((long long)chunks * slice / g.slices), cs1 = (int)((long long)chunks * (slice + 1) / g.slices);
+    const int kEnd = min(K, cs1 * 16);
+    const int iters = (cs1 - cs0 + 3) / 4;  // chunks of this wave: cs0 + wave, cs0 + wave + 4, ...
 
     double ea = 0.0, eb = 0.0;  // this lane's share of |op(A)_i|^2 and |op(B)_j|^2
     auto load = [&](SmallSet& S, int it) {
@@ -72,11 +83,11 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         for (int c = 0; c < kSmallG; ++c)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int k = ((it * kSmallG + c) * 4 + wave) * 16 + 4 * t + u;
-                const int kc = min(k, K - 1);
+                const int k = (cs0 + (it * kSmallG + c) * 4 + wave) * 16 + 4 * t + u;
+                const int kc = min(k, kEnd - 1);
                 float2 a = A[aFix + (long long)kc * aStep];
                 float2 b = B[bFix + (long long)kc * bStep];
-                const bool kIn = k < K;
+                const bool kIn = k < kEnd;
                 a.x = (kIn && iIn) ? a.x : 0.0f;
                 a.y = (kIn && iIn) ? a.y * aSign : 0.0f;
                 b.x = (kIn && jIn) ? b.x : 0.0f;
@@ -144,13 +155,63 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         if (tid == 0) flagged = 0;
     }
     __syncthreads();
+    float p[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) p[q] = ((red[0][q][tid] + red[1][q][tid]) + red[2][q][tid]) + red[3][q][tid];
+    double eAll = 0.0;  // slices > 1: thread t < 16 row t's energy, 16 <= t < 32 column t-16's
+    if (g.slices > 1) {
+        // Publish this slice and count in; only the last to arrive goes on. The tiles' slices
+        // run on different XCDs, whose L2 caches are not coherent with one another: the
+        // partials are written and read with agent-scope atomic stores and loads (coherent
+        // across the L2s, no cache flush), and the stores have completed (vmcnt 0) before
+        // the workgroup's barrier and its arrival. A device-scope fence would instead write
+        // back and invalidate this XCD's whole L2 in every workgroup.
+        float* mine = g.part + ((size_t)tile * g.slices + slice) * 768;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            __hip_atomic_store(mine + q * 256 + tid, p[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g.correction && tid < 32) {
+            const int r = tid & 15, side = tid >> 4;
+            __hip_atomic_store(g.epart + ((size_t)tile * g.slices + slice) * 32 + tid,
+                               ((ered[0][side][r] + ered[1][side][r]) + ered[2][side][r]) + ered[3][side][r],
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            const unsigned int prev =
+                __hip_atomic_fetch_add(&g.arrive[tile], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lastIn = prev == (unsigned int)(g.slices - 1);
+            if (lastIn)  // ready for the next launch
+                __hip_atomic_store(&g.arrive[tile], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (!lastIn) return;
+        const float* all = g.part + (size_t)tile * g.slices * 768;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            float v = __hip_atomic_load(all + q * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int sl = 1; sl < g.slices; ++sl)
+                v += __hip_atomic_load(all + (size_t)sl * 768 + q * 256 + tid, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            p[q] = v;
+        }
+        if (g.correction && tid < 32) {
+            const double* ep = g.epart + (size_t)tile * g.slices * 32;
+            double v = __hip_atomic_load(ep + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int sl = 1; sl < g.slices; ++sl)
+                v += __hip_atomic_load(ep + (size_t)sl * 32 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            eAll = v;
+        }
+        if (g.correction) {
+            if (tid < 32) ered[0][tid >> 4][tid & 15] = eAll;
+            __syncthreads();
+        }
+    }
     const int io = i0 + (tid & 15), jo = j0 + (tid >> 4);
     float2* C = reinterpret_cast<float2*>(g.C);
     const bool bz = g.betaZero != 0;
     if (io < g.m && jo < g.n) {
-        float p[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) p[q] = ((red[0][q][tid] + red[1][q][tid]) + red[2][q][tid]) + red[3][q][tid];
         float re, im;
         combine3(g.identity, p[0], p[1], p[2], re, im);
         const size_t o = (size_t)jo * g.ldc + io;
@@ -160,8 +221,10 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         bool flag = false;
         if (g.correction) {
             const int il = tid & 15, jl = tid >> 4;
-            const double eA = ((ered[0][0][il] + ered[1][0][il]) + ered[2][0][il]) + ered[3][0][il];
-            const double eB = ((ered[0][1][jl] + ered[1][1][jl]) + ered[2][1][jl]) + ered[3][1][jl];
+            const double eA = g.slices > 1 ? ered[0][0][il]
+                                           : ((ered[0][0][il] + ered[1][0][il]) + ered[2][0][il]) + ered[3][0][il];
+            const double eB = g.slices > 1 ? ered[0][1][jl]
+                                           : ((ered[0][1][jl] + ered[1][1][jl]) + ered[2][1][jl]) + ered[3][1][jl];
             flag = needs_recompute(c, eA, eB, g.corrScale, g.corrTau);
         }
         if (flag) {
@@ -184,7 +247,7 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
                                                        make_float2(g.alphaR, g.alphaI), make_float2(g.betaR, g.betaI),
                                                        bz, flagC0[e]);
         }
-        if (tid == 0 && g.flagCount != nullptr) g.flagCount[blockIdx.x] = (unsigned int)count;
+        if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = (unsigned int)count;
         return;
     }
     for (int e = 0; e < count; ++e) {
@@ -224,13 +287,22 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         }
         __syncthreads();
     }
-    if (tid == 0 && g.flagCount != nullptr) g.flagCount[blockIdx.x] = (unsigned int)count;
+    if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = (unsigned int)count;
 }
 
+// g.arrive must hold zeros when g.slices > 1 (launch_fill_u32 before the first such launch on
+// a buffer; every launch leaves them zero).
 hipError_t launch_gemm_small(const SmallArgs& g, hipStream_t s)
 {
     const int tilesM = (g.m + 15) / 16, tilesN = (g.n + 15) / 16;
-    hipLaunchKernelGGL(gemm3m_small_kernel, dim3(tilesM * tilesN), dim3(256), 0, s, g, tilesM);
+    const long long tiles = (long long)tilesM * tilesN;
+    const int chunks = (g.k + 15) / 16;
+    if (g.slices < 1 || (g.slices > 1 && (g.slices > chunks || g.part == nullptr || g.arrive == nullptr ||
+                                          (g.correction && g.epart == nullptr))) ||
+        tiles * g.slices > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gemm3m_small_kernel, dim3((unsigned)(tiles * g.slices)), dim3(256), 0, s, g, tilesM,
+                       (int)tiles);
     return hipGetLastError();
 }
 

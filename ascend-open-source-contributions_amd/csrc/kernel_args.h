@@ -116,7 +116,13 @@ struct SmallArgs {      // one-launch path for small problems: complex inputs re
     int correction;                         // screen + in-kernel recompute of flagged outputs:
                                             // 1 FP64, 2 the reference BLAS's FP32 order
     double corrScale, corrTau;
-    unsigned int* flagCount;                // optional: flagged outputs per workgroup (one slot each)
+    unsigned int* flagCount;                // optional: flagged outputs per output tile (one slot each)
+    // K split inside the launch: slices > 1 workgroups per 16x16 tile, each over its own run of
+    // 16-deep k chunks; the last to arrive adds the slices' partials in slice order.
+    int slices;
+    float* part;                            // [tile][slice][3][256] partial products
+    double* epart;                          // [tile][slice][32] partial row / column energies
+    unsigned int* arrive;                   // [tile] arrival counters, zero at launch (left zero)
 };
 
 struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/beta

@@ -385,6 +385,33 @@ def test_small_kernel_ops(gpu, ta, tb, identity):
     assert plan["path"] == 5
 
 
+@pytest.mark.parametrize("slices", [2, 3, 7, 16])
+def test_small_kernel_k_slices(gpu, slices):
+    """The small-problem kernel with its k range split over `slices` workgroups per tile (the
+    last to arrive adds the slices in slice order): a ragged tall-skinny shape with padded
+    leading dimensions, beta != 0, with the correction (k >= 1024: energies summed over the
+    slices, flagged outputs recomputed) and without it; two calls in a row (the arrival
+    counters are left zero for the next launch), byte-identical reruns."""
+    import cgemm3m
+
+    for (m, n, k, ta, tb), correction in (((181, 37, 1500, "T", "N"), 1), ((90, 70, 700, "N", "C"), -1)):
+        lda = (m if ta == "N" else k) + 3
+        ldb = (k if tb == "N" else n) + 2
+        cfg = cgemm3m.config(variant=5, k_split=slices, correction=correction)
+        out, plan = _case(gpu, ta, tb, m, n, k, ALPHA2, BETA2, lda, ldb, m + 1, 20260900 + slices + k, cfg=cfg,
+                          label="small-sliced%d" % slices)
+        assert plan["path"] == 5 and plan["parts"] == slices, plan
+        assert (plan["flagged"] >= 0) == (correction == 1), plan
+
+
+def test_small_kernel_auto_slices(gpu):
+    """Few 16x16 tiles and a long k: the planner slices k on its own (about two workgroups
+    per CU)."""
+    m, n, k = 464, 21, 1663
+    out, plan = _case(gpu, "N", "T", m, n, k, ALPHA2, BETA2, m, n, m, 20260950, label="small-auto-sliced")
+    assert plan["path"] == 5 and plan["parts"] > 1, plan
+
+
 @pytest.mark.parametrize("shape", [(200, 150, 300), (1, 1, 1), (33, 17, 1000), (64, 300, 5)])
 def test_small_kernel_auto(gpu, shape):
     """Shapes the planner sends to the small-problem kernel by default (correction off,
