@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the torch.matmul complex64 comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3 counter passes: 3 NN calls of the first --variants form (fp32 "
@@ -161,6 +162,10 @@ def main():
             log("row-scaled %s err %s" % (name, res))
         del As
 
+    if not args.no_vendor and "NN" in args.ops.split(","):
+        results["torch_matmul_complex64"] = vendor_bench(h, torch, cgemm3m, A, B, C0, m, n, k, args)
+        log("torch.matmul complex64", results["torch_matmul_complex64"])
+
     if not args.no_grouped:
         results["grouped_512"] = grouped_bench(h, torch, cgemm3m, dev)
         log("grouped", results["grouped_512"])
@@ -205,6 +210,32 @@ def main():
         "variants": results,
     }
     print(json.dumps(line))
+
+
+def vendor_bench(h, torch, cgemm3m, A, B, C0, m, n, k, args):
+    """The same product through torch.matmul on complex64 (the ROCm BLAS library's complex GEMM,
+    four real products, no alpha/beta): C = A B in column-major storage is C^T = B^T A^T on the
+    row-major views, so no operand is transposed or copied. A reported comparison on the same
+    GPU, not a baseline of the metric."""
+    P = cgemm3m.torch_ptr
+    Bt, At = B.view(n, k), A.view(k, m)
+    out = torch.empty(n, m, dtype=torch.complex64, device=A.device)
+    for _ in range(2):
+        torch.matmul(Bt, At, out=out)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    steps = max(3, args.steps // 2)
+    ev0.record()
+    for _ in range(steps):
+        torch.matmul(Bt, At, out=out)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    r = {"ms_per_step": ms, "tflops": 8.0 * m * n * k / (ms * 1e-3) / 1e12,
+         "op": "torch.matmul complex64 (N,N), alpha=1, beta=0"}
+    if not args.no_errors:
+        r["err_vs_fp64"] = cgemm3m.residual(h, "N", "N", m, n, k, 1.0, P(A), m, P(B), k, 0.0, P(C0), P(out), m)
+    return r
 
 
 def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
