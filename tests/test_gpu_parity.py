@@ -317,13 +317,15 @@ def test_correction_fixes_flagged_and_counts(gpu):
     assert abs(complex(on[0]) - z[0]) <= 1e-6 * abs(z[0]) + 1e-30
 
 
-@pytest.mark.parametrize("pin", [dict(), dict(tile=64, k_split=1), dict(tile=128, k_split=2)])
+@pytest.mark.parametrize("pin", [dict(), dict(variant=5, k_split=1), dict(variant=5, k_split=4),
+                                 dict(tile=64, k_split=1), dict(tile=128, k_split=2)])
 def test_correction_in_reference_blas_order(gpu, pin):
     """correction = 2: flagged outputs are recomputed in the reference BLAS's sequential FP32
     order, so they are bit-identical to the Netlib-order oracle; the rest meet the mixed
     tolerance.
-    Planner's choice (the one-launch kernel, in-kernel recompute), the packed tile path
-    (recompute kernel) and the K-split path (flags raised by the merge)."""
+    Planner's choice, the one-launch kernel unsliced and with k in four slices (in-kernel
+    recompute by the tile's last workgroup), the packed tile path (recompute kernel) and the
+    K-split path (flags raised by the merge)."""
     import cgemm3m
 
     m, n, k = 64, 64, 4096
