@@ -50,8 +50,9 @@ def main():
                          "WRITE_SIZE), nothing timed")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the operator shards by output blocks with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path here (across GPUs the "
+              "operator would shard C by output blocks, each GPU holding its rows of op(A) and columns "
+              "of op(B), with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import torch
