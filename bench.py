@@ -43,6 +43,7 @@ def main():
     ap.add_argument("--no-errors", action="store_true")
     ap.add_argument("--no-grouped", action="store_true")
     ap.add_argument("--no-vendor", action="store_true", help="skip the torch.matmul complex64 comparison")
+    ap.add_argument("--no-autograd", action="store_true", help="skip the autograd (forward + backward) timing")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--profile-run", action="store_true",
                     help="for rocprofv3 counter passes: 3 NN calls of the first --variants form (fp32 "
@@ -167,6 +168,10 @@ def main():
         results["torch_matmul_complex64"] = vendor_bench(h, torch, cgemm3m, A, B, C0, m, n, k, args)
         log("torch.matmul complex64", results["torch_matmul_complex64"])
 
+    if not args.no_autograd:
+        results["autograd_2048x1024x4096"] = autograd_bench(torch, cgemm3m, dev, args)
+        log("autograd", results["autograd_2048x1024x4096"])
+
     if not args.no_grouped:
         results["grouped_512"] = grouped_bench(h, torch, cgemm3m, dev)
         log("grouped", results["grouped_512"])
@@ -237,6 +242,51 @@ def vendor_bench(h, torch, cgemm3m, A, B, C0, m, n, k, args):
     if not args.no_errors:
         r["err_vs_fp64"] = cgemm3m.residual(h, "N", "N", m, n, k, 1.0, P(A), m, P(B), k, 0.0, P(C0), P(out), m)
     return r
+
+
+def autograd_bench(torch, cgemm3m, dev, args, m=2048, k=1024, n=4096, seed=777):
+    """The autograd configuration: Y = alpha A B + beta C, A m x k, B k x n, C m x n complex64
+    (re/im uniform[-1,1]), alpha = (0.8, 0.6), beta = (0.5, 0); one step = the forward and the
+    backward (dA, dB, dC) through the Cgemm3m kernels (FP32 and bf16x3), and the same step
+    through torch.matmul autograd. Work per step: three complex products, 3 x 8mnk."""
+    from cgemm3m.autograd import cgemm3m_mm
+
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+
+    def u(*shape):
+        return (torch.rand(*shape, 2, generator=gen, device=dev) * 2 - 1).view(torch.complex64).reshape(shape)
+
+    A, B, C, gY = u(m, k), u(k, n), u(m, n), u(m, n)
+    alpha, beta = complex(0.8, 0.6), complex(0.5, 0.0)
+    out = {}
+    forms = {
+        "fp32": lambda Ar, Br, Cr: cgemm3m_mm(Ar, Br, Cr, alpha, beta, "fp32"),
+        "bf16x3": lambda Ar, Br, Cr: cgemm3m_mm(Ar, Br, Cr, alpha, beta, "bf16x3"),
+        "torch_matmul": lambda Ar, Br, Cr: alpha * (Ar @ Br) + beta * Cr,
+    }
+    steps = max(3, args.steps // 2)
+    for name, f in forms.items():
+        Ar, Br, Cr = (x.clone().requires_grad_(True) for x in (A, B, C))
+
+        def step():
+            for x in (Ar, Br, Cr):
+                x.grad = None
+            f(Ar, Br, Cr).backward(gY)
+
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(steps):
+            step()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        out[name] = {"ms_per_step": ms, "tflops": 3 * 8.0 * m * n * k / (ms * 1e-3) / 1e12}
+    out["step"] = "forward + backward of alpha*A@B + beta*C, A %dx%d, B %dx%d" % (m, k, k, n)
+    return out
 
 
 def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
