@@ -2,8 +2,8 @@
 and writes csrc/tuned_table.h (compiled into the library).
 
 A class is the bin (floor(log2(m*n)), floor(log2(k))). One-problem candidates: output tile 64
-or 128 with 1-16 K partitions, and the one-launch small-problem kernel ("tile 32") with 1-16 k
-slices per output tile. Grouped classes are timed as one aclblasCgemm3mGroupedBatchedEx call
+or 128 with 1-16 K partitions, and the one-launch small-problem kernel with four waves per
+16x16 output tile ("tile 32") or one ("tile 16"), with 1-16 k slices per output tile. Grouped classes are timed as one aclblasCgemm3mGroupedBatchedEx call
 of identical problems with the member plan pinned. The table is deterministic input to the
 planner: the same arguments always get the same plan.
 
@@ -48,19 +48,21 @@ def bin_of(m, n, k):
 
 
 def candidates(m, n, k):
-    """(tile, parts) plans worth timing for one problem (tile 32: small kernel, parts = slices)."""
+    """(tile, parts) plans worth timing for one problem (tile 32 / 16: the one-launch kernel with
+    four waves / one wave per output tile, parts = k slices)."""
     steps = (k + 31) // 32 * 2          # 16-deep stages of the packed kernels
     chunks = (k + 15) // 16             # 16-deep k chunks of the small kernel
     out = [(tile, parts) for tile in (64, 128) for parts in (1, 2, 4, 8, 16)
            if parts == 1 or steps // parts >= 2]
     if m * n <= 2048 * 2048:
         out += [(32, s) for s in (1, 2, 4, 8, 16) if s == 1 or chunks // s >= 4]
+        out += [(16, s) for s in (1, 2, 4) if s == 1 or chunks // s >= 2]
     return out
 
 
 def _config(cgemm3m, tile, parts):
-    if tile == 32:
-        return cgemm3m.config("fp32", variant=5, k_split=parts)
+    if tile in (16, 32):
+        return cgemm3m.config("fp32", variant=5, k_split=parts, tile=16 if tile == 16 else 0)
     return cgemm3m.config("fp32", tile=tile, k_split=parts)
 
 
@@ -252,7 +254,8 @@ def write_table(entries, path=TABLE):
     lines = [
         "// tuned_table.h -- plan table written by cgemm3m/autotune.py from MI355X measurements.",
         "// Key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and K partitions",
-        "// (tile 32: the one-launch small-problem kernel).",
+        "// (tile 32 / 16: the one-launch small-problem kernel, four waves / one wave per 16x16 tile;",
+        "// parts = k slices).",
         "// Bins not listed fall back to the planner's occupancy rule.",
         "#pragma once",
         "",

@@ -168,16 +168,17 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     auto tiles_for = [&](int t) { return ((m + t - 1) / t) * ((n + t - 1) / t); };
     int tile = 0;
     if (cfg && cfg->tile != 0) {
-        if (cfg->tile != 64 && cfg->tile != 128) {
+        // tile 16 names the one-launch kernel's single-wave form (variant 5)
+        if (cfg->tile != 64 && cfg->tile != 128 && !(cfg->tile == 16 && cfg->variant == 5)) {
             log_error(op, "invalid tile %d", cfg->tile);
             return false;
         }
-        tile = cfg->tile;
+        tile = cfg->tile == 16 ? 64 : cfg->tile;
     } else {
         // The tuned table holds FP32-product measurements; other formats use the occupancy rule.
-        // (An entry naming the small-problem kernel, tile 32, is used further down.)
+        // (An entry naming the one-launch kernel, tile 32 or 16, is used further down.)
         int tt = 0, tp = 0;
-        if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp) && tt != 32)
+        if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp) && tt >= 64)
             tile = tt;
         else
             tile = tiles_for(128) >= cus ? 128 : 64;
@@ -195,7 +196,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         }
         pl.parts = std::min(cfg->kSplit, pl.kSteps);
     } else if (int tt = 0, tp = 0; pl.format == FMT_F32 && (!cfg || cfg->tile == 0) &&
-                                   tuned_lookup(m, n, k, false, &tt, &tp) && tt != 32) {
+                                   tuned_lookup(m, n, k, false, &tt, &tp) && tt >= 64) {
         pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
         // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition.
@@ -259,16 +260,17 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     else
         pl.correction = k >= 1024;
     // One-launch small-problem kernel: no pack launch. Pinned with variant 5 (kSplit = k
-    // slices per tile), or chosen when nothing is pinned if the tuned table names it (tile 32,
-    // parts = slices) for this shape class or, for classes it does not list, the output has
-    // at most 256 of its 16x16 tiles. Workspace: none, unless the correction is on (one
-    // flagged-output counter per tile) or k is sliced (arrival counters, partial products and
-    // energies per slice).
+    // slices per tile, tile 16 = one wave per tile instead of four), or chosen when nothing is
+    // pinned if the tuned table names it (tile 32 four waves, tile 16 one wave; parts =
+    // slices) for this shape class or, for classes it does not list, the output has at most
+    // 256 of its 16x16 tiles. Workspace: none, unless the correction is on (one flagged-output
+    // counter per tile) or k is sliced (arrival counters, partial products and energies per
+    // slice).
     {
         int tt = 0, tp = 0;
         const bool listed = tuned_lookup(m, n, k, false, &tt, &tp);
         const long long tiles16 = ((m + 15) / 16) * ((n + 15) / 16);
-        const bool fits = listed ? tt == 32 : tiles16 <= 256;
+        const bool fits = listed ? (tt == 32 || tt == 16) : tiles16 <= 256;
         const bool autoSmall = pl.format == FMT_F32 && !pinned && pl.identity != ID_FOUR && fits;
         if ((cfg && cfg->variant == 5) || autoSmall) {
             const int correction = pl.correction;
@@ -290,6 +292,9 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
             pl.variant = 4;
             pl.correction = correction;
             pl.smallTiles = tiles16;
+            // waves per tile: one for the single-wave form (table tile 16, or pinned tile 16)
+            const bool single = (cfg && cfg->variant == 5) ? cfg->tile == 16 : (listed && tt == 16);
+            pl.smallWaves = single ? 1 : 4;
             size_t total = 0, bytes = 0, n2 = 0;
             if (slices > 1) {
                 // whole 8-byte words: the counters are cleared as 64-bit words
@@ -389,7 +394,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         const bool sliced = pl.parts > 1;
         SmallArgs a{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.m, p.n, p.k, pl.identity,
                     p.alpha.real, p.alpha.imag, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc,
-                    pl.correction, amag * 0x1p-24, 0x1p-8, slots, pl.parts,
+                    pl.correction, amag * 0x1p-24, 0x1p-8, slots, pl.parts, pl.smallWaves,
                     sliced ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr,
                     sliced && pl.correction ? reinterpret_cast<double*>(ws + pl.offEPA) : nullptr,
                     sliced ? reinterpret_cast<unsigned int*>(ws + pl.offArrive) : nullptr};

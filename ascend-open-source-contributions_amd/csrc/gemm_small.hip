@@ -3,9 +3,11 @@
 // For small outputs the pack + product launches and a long per-wave MFMA chain set the time,
 // not throughput. This kernel reads the caller's interleaved complex64 A and B directly (no
 // workspace, no pack launch) and forms the three 3M plane values in registers. One workgroup
-// computes one 16x16 output tile; its four waves split k (wave w takes the 16-deep k chunks
-// w, w+4, w+8, ...), so a wave's MFMA chain is a quarter of the k range, and the four partial
-// products are added in LDS in wave order (deterministic) before the alpha/beta epilogue.
+// computes one 16x16 output tile with W waves: with W = 4 they split k (wave w takes the
+// 16-deep k chunks w, w+4, w+8, ...), so a wave's MFMA chain is a quarter of the k range, and
+// the four partial products are added in LDS in wave order (deterministic) before the
+// alpha/beta epilogue; with W = 1 (short k and many output tiles) the one wave takes every
+// chunk and the workgroup is a quarter the size.
 // Reference role: the on-chip short-K path (ops-blas/blas/gemm/arch35/cgemm_onchip_kernel.h:
 // 84-353, which deinterleaves into on-chip buffers and needs no workspace), re-designed for
 // MFMA fragments.
@@ -45,11 +47,14 @@ struct SmallSet {
 
 }  // namespace
 
-__global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int tilesM, int tiles)
+template <int W>
+__global__ __launch_bounds__(64 * W, W == 4 ? 2 : 8) void gemm3m_small_kernel(SmallArgs g, int tilesM, int tiles)
 {
-    __shared__ float red[4][3][256];
-    __shared__ double ered[4][2][16];  // per wave: row energies of op(A), column energies of op(B)
-    __shared__ double dred[4][2];
+    constexpr int NT = 64 * W;
+    __shared__ float red[W][3][256];
+    __shared__ double ered[W][2][16];  // per wave: row energies of op(A), column energies of op(B)
+    __shared__ double eTot[2][16];     // the tile's row and column energies (all waves, all slices)
+    __shared__ double dred[W][2];
     __shared__ int flagged;
     __shared__ int flagList[256];
     __shared__ float2 flagC0[256];
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     // this slice's chunks [cs0, cs1) (at least one: the planner keeps slices <= chunks)
     const int cs0 = (int)((long long)chunks * slice / g.slices), cs1 = (int)((long long)chunks * (slice + 1) / g.slices);
     const int kEnd = min(K, cs1 * 16);
-    const int iters = (cs1 - cs0 + 3) / 4;  // chunks of this wave: cs0 + wave, cs0 + wave + 4, ...
+    const int iters = (cs1 - cs0 + W - 1) / W;  // chunks of this wave: cs0 + wave, cs0 + wave + W, ...
 
     double ea = 0.0, eb = 0.0;  // this lane's share of |op(A)_i|^2 and |op(B)_j|^2
     // An operand stored k-contiguous (A as T/C, B as N) gives each lane its four consecutive k
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     auto load = [&](SmallSet& S, int it) {
 #pragma unroll
         for (int c = 0; c < kSmallG; ++c) {
-            const int k0 = (cs0 + (it * kSmallG + c) * 4 + wave) * 16 + 4 * t;
+            const int k0 = (cs0 + (it * kSmallG + c) * W + wave) * 16 + 4 * t;
             float2 a[4], b[4];
             load4(A, aFix, aStep, !aN, k0, a);
             load4(B, bFix, bStep, bN, k0, b);
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     }
     if (it < sets) mfma(s0);
 
-    // Partial products of the four waves, added in wave order.
+    // Partial products of the W waves (added in wave order below).
 #pragma unroll
     for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -175,10 +180,20 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         if (tid == 0) flagged = 0;
     }
     __syncthreads();
-    float p[3];
+    // thread t < 32: the tile's energy of row t (t < 16) or column t - 16, waves in order
+    if (g.correction && tid < 32) {
+        const int side = tid >> 4, r = tid & 15;
+        double v = ered[0][side][r];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) p[q] = ((red[0][q][tid] + red[1][q][tid]) + red[2][q][tid]) + red[3][q][tid];
-    double eAll = 0.0;  // slices > 1: thread t < 16 row t's energy, 16 <= t < 32 column t-16's
+        for (int w = 1; w < W; ++w) v += ered[w][side][r];
+        eTot[side][r] = v;
+    }
+    auto wave_sum = [&](int q, int o) {
+        float v = red[0][q][o];
+#pragma unroll
+        for (int w = 1; w < W; ++w) v += red[w][q][o];
+        return v;
+    };
     if (g.slices > 1) {
         // Publish this slice and count in; only the last to arrive goes on. The tiles' slices
         // run on different XCDs, whose L2 caches are not coherent with one another: the
@@ -187,15 +202,13 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         // the workgroup's barrier and its arrival. A device-scope fence would instead write
         // back and invalidate this XCD's whole L2 in every workgroup.
         float* mine = g.part + ((size_t)tile * g.slices + slice) * 768;
+        for (int o = tid; o < 256; o += NT)
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
-            __hip_atomic_store(mine + q * 256 + tid, p[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (g.correction && tid < 32) {
-            const int r = tid & 15, side = tid >> 4;
-            __hip_atomic_store(g.epart + ((size_t)tile * g.slices + slice) * 32 + tid,
-                               ((ered[0][side][r] + ered[1][side][r]) + ered[2][side][r]) + ered[3][side][r],
+            for (int q = 0; q < 3; ++q)
+                __hip_atomic_store(mine + q * 256 + o, wave_sum(q, o), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g.correction && tid < 32)
+            __hip_atomic_store(g.epart + ((size_t)tile * g.slices + slice) * 32 + tid, eTot[tid >> 4][tid & 15],
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (tid == 0) {
@@ -207,52 +220,48 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         }
         __syncthreads();
         if (!lastIn) return;
+        // every slice's partials, added in slice order, into red[0] (its wave sums were
+        // published above and are not read again)
         const float* all = g.part + (size_t)tile * g.slices * 768;
+        for (int o = tid; o < 256; o += NT)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            float v = __hip_atomic_load(all + q * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            for (int sl = 1; sl < g.slices; ++sl)
-                v += __hip_atomic_load(all + (size_t)sl * 768 + q * 256 + tid, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            p[q] = v;
-        }
+            for (int q = 0; q < 3; ++q) {
+                float v = __hip_atomic_load(all + q * 256 + o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int sl = 1; sl < g.slices; ++sl)
+                    v += __hip_atomic_load(all + (size_t)sl * 768 + q * 256 + o, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                red[0][q][o] = v;
+            }
         if (g.correction && tid < 32) {
             const double* ep = g.epart + (size_t)tile * g.slices * 32;
             double v = __hip_atomic_load(ep + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (int sl = 1; sl < g.slices; ++sl)
                 v += __hip_atomic_load(ep + (size_t)sl * 32 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            eAll = v;
-        }
-        if (g.correction) {
-            if (tid < 32) ered[0][tid >> 4][tid & 15] = eAll;
-            __syncthreads();
+            eTot[tid >> 4][tid & 15] = v;
         }
     }
-    const int io = i0 + (tid & 15), jo = j0 + (tid >> 4);
+    __syncthreads();
     float2* C = reinterpret_cast<float2*>(g.C);
     const bool bz = g.betaZero != 0;
-    if (io < g.m && jo < g.n) {
+    for (int o = tid; o < 256; o += NT) {
+        const int io = i0 + (o & 15), jo = j0 + (o >> 4);
+        if (io >= g.m || jo >= g.n) continue;
+        float p[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) p[q] = g.slices > 1 ? red[0][q][o] : wave_sum(q, o);
         float re, im;
         combine3(g.identity, p[0], p[1], p[2], re, im);
-        const size_t o = (size_t)jo * g.ldc + io;
+        const size_t co = (size_t)jo * g.ldc + io;
         float2 c0 = make_float2(0.0f, 0.0f);
-        if (!bz) c0 = C[o];
+        if (!bz) c0 = C[co];
         const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0);
-        bool flag = false;
-        if (g.correction) {
-            const int il = tid & 15, jl = tid >> 4;
-            const double eA = g.slices > 1 ? ered[0][0][il]
-                                           : ((ered[0][0][il] + ered[1][0][il]) + ered[2][0][il]) + ered[3][0][il];
-            const double eB = g.slices > 1 ? ered[0][1][jl]
-                                           : ((ered[0][1][jl] + ered[1][1][jl]) + ered[2][1][jl]) + ered[3][1][jl];
-            flag = needs_recompute(c, eA, eB, g.corrScale, g.corrTau);
-        }
+        const bool flag = g.correction && needs_recompute(c, eTot[0][o & 15], eTot[1][o >> 4], g.corrScale, g.corrTau);
         if (flag) {
             const int slot = atomicAdd(&flagged, 1);
-            flagList[slot] = tid;
+            flagList[slot] = o;
             flagC0[slot] = c0;
         } else {
-            C[o] = c;
+            C[co] = c;
         }
     }
     if (!g.correction) return;
@@ -260,7 +269,7 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
     const int count = flagged;
     if (g.correction == 2) {
         // the reference BLAS's sequential FP32 order: one flagged output per thread
-        for (int e = tid; e < count; e += 256) {
+        for (int e = tid; e < count; e += NT) {
             const int ft = flagList[e];
             const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);
             C[(size_t)fj * g.ldc + fi] = netlib_output(A, g.lda, g.opA, B, g.ldb, g.opB, fi, fj, K,
@@ -274,7 +283,7 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         const int ft = flagList[e];
         const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);
         double sr = 0.0, si = 0.0;
-        for (long long l = tid; l < K; l += 256) {
+        for (long long l = tid; l < K; l += NT) {
             const float2 x = aN ? A[(size_t)l * g.lda + fi] : A[(size_t)fi * g.lda + l];  // op(A)(fi, l)
             const float2 y = bN ? B[(size_t)fj * g.ldb + l] : B[(size_t)l * g.ldb + fj];  // op(B)(l, fj)
             const double xr = x.x, xi = aSign * (double)x.y, yr = y.x, yi = bSign * (double)y.y;
@@ -294,8 +303,12 @@ __global__ __launch_bounds__(256, 2) void gemm3m_small_kernel(SmallArgs g, int t
         }
         __syncthreads();
         if (tid == 0) {
-            const double tr = ((dred[0][0] + dred[1][0]) + dred[2][0]) + dred[3][0];
-            const double ti = ((dred[0][1] + dred[1][1]) + dred[2][1]) + dred[3][1];
+            double tr = dred[0][0], ti = dred[0][1];
+#pragma unroll
+            for (int w = 1; w < W; ++w) {
+                tr += dred[w][0];
+                ti += dred[w][1];
+            }
             double cr = g.alphaR * tr - g.alphaI * ti;
             double ci = g.alphaR * ti + g.alphaI * tr;
             if (!bz) {
@@ -321,8 +334,14 @@ hipError_t launch_gemm_small(const SmallArgs& g, hipStream_t s)
                                           (g.correction && g.epart == nullptr))) ||
         tiles * g.slices > 0x7fffffffLL)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(gemm3m_small_kernel, dim3((unsigned)(tiles * g.slices)), dim3(256), 0, s, g, tilesM,
-                       (int)tiles);
+    if (g.waves == 1)
+        hipLaunchKernelGGL(gemm3m_small_kernel<1>, dim3((unsigned)(tiles * g.slices)), dim3(64), 0, s, g, tilesM,
+                           (int)tiles);
+    else if (g.waves == 4)
+        hipLaunchKernelGGL(gemm3m_small_kernel<4>, dim3((unsigned)(tiles * g.slices)), dim3(256), 0, s, g, tilesM,
+                           (int)tiles);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ struct Plan {
     int format = FMT_F32;
     int pieces = 1;
     long long smallTiles = 0;  // path 5: 16x16 output tiles (one flagged-output slot each)
+    int smallWaves = 4;        // path 5: waves per output tile (4, or 1 for the single-wave form)
     int bm = 128, bn = 128;
     int bk = 16;          // k depth of one GEMM stage
     int parts = 1;

@@ -120,6 +120,7 @@ struct SmallArgs {      // one-launch path for small problems: complex inputs re
     // K split inside the launch: slices > 1 workgroups per 16x16 tile, each over its own run of
     // 16-deep k chunks; the last to arrive adds the slices' partials in slice order.
     int slices;
+    int waves;                              // waves per tile: 4 (each a quarter of the chunks) or 1
     float* part;                            // [tile][slice][3][256] partial products
     double* epart;                          // [tile][slice][32] partial row / column energies
     unsigned int* arrive;                   // [tile] arrival counters, zero at launch (left zero)

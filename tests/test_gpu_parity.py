@@ -407,6 +407,23 @@ def test_small_kernel_k_slices(gpu, slices):
         assert (plan["flagged"] >= 0) == (correction == 1), plan
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "C"), ("C", "T")])
+def test_small_kernel_single_wave(gpu, ta, tb):
+    """The one-launch kernel's single-wave form (tile 16: one wave per 16x16 tile takes every k
+    chunk, 64-thread workgroups): ragged short-k and long-k shapes with padded leading
+    dimensions, unsliced and with k in three slices, the correction on (k >= 1024) and off."""
+    import cgemm3m
+
+    for (m, n, k), slices in (((100, 70, 40), 1), ((181, 37, 1500), 1), ((181, 37, 1500), 3)):
+        lda = (m if ta == "N" else k) + 3
+        ldb = (k if tb == "N" else n) + 2
+        cfg = cgemm3m.config(variant=5, tile=16, k_split=slices)
+        out, plan = _case(gpu, ta, tb, m, n, k, ALPHA2, BETA2, lda, ldb, m + 1,
+                          20261600 + 3 * "NTC".index(ta) + "NTC".index(tb) + k + slices, cfg=cfg,
+                          label="small-1wave-s%d" % slices)
+        assert plan["path"] == 5 and plan["parts"] == slices, plan
+
+
 def test_small_kernel_auto_slices(gpu):
     """Few 16x16 tiles and a long k: the planner slices k on its own (about two workgroups
     per CU)."""
