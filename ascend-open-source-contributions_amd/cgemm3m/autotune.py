@@ -40,7 +40,9 @@ SINGLE_SHAPES = [
     (64, 64, 64), (128, 128, 128), (128, 128, 512), (384, 384, 384), (512, 512, 128), (768, 768, 768),
     (1024, 1024, 256), (1024, 1024, 512),
 ]
-GROUPED_SHAPES = [(m, m, k) for m in (32, 64, 128, 256, 512) for k in (64, 256, 1024, 2048)]
+# square members (even log2 m*n classes) and m x 2m members (the odd classes between them)
+GROUPED_SHAPES = [(m, m * r, k) for m in (32, 64, 128, 256, 512) for r in (1, 2) if m * r <= 512
+                  for k in (64, 256, 1024, 2048)]
 
 
 def bin_of(m, n, k):
@@ -239,6 +241,10 @@ def tune_grouped(h, torch, shapes, reps, count=64):
                     continue
                 cfg = cgemm3m.config("fp32", tile=tile, k_split=parts)
                 res[(tile, parts)] = timed(lambda: cgemm3m.grouped(h, probs, cfg=cfg), reps)
+        if m * n <= 512 * 512:  # the one-launch kernel, four waves / one wave per tile, unsliced
+            for tile in (32, 16):
+                cfg = cgemm3m.config("fp32", variant=5, tile=16 if tile == 16 else 0, k_split=1)
+                res[(tile, 1)] = timed(lambda: cgemm3m.grouped(h, probs, cfg=cfg), reps)
         best = choose(res)
         print("grouped %4d x %4d x %5d  best tile %d parts %d  (all: %s)" % (
             m, n, k, best[0], best[1], {"%d/%d" % kk: round(v, 3) for kk, v in res.items()}), flush=True)

@@ -47,8 +47,10 @@ struct SmallSet {
 
 }  // namespace
 
+// Workgroup `block` of a launch over `tiles` output tiles x g.slices: tile block % tiles,
+// k slice block / tiles.
 template <int W>
-__global__ __launch_bounds__(64 * W, W == 4 ? 2 : 8) void gemm3m_small_kernel(SmallArgs g, int tilesM, int tiles)
+__device__ __forceinline__ void small_tile(const SmallArgs& g, int tilesM, int tiles, int block)
 {
     constexpr int NT = 64 * W;
     __shared__ float red[W][3][256];
@@ -61,7 +63,7 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 2 : 8) void gemm3m_small_kernel(Sm
     __shared__ int lastIn;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = blockIdx.x % tiles, slice = blockIdx.x / tiles;
+    const int tile = block % tiles, slice = block / tiles;
     const int tm = tile % tilesM, tn = tile / tilesM;
     const int i0 = tm * 16, j0 = tn * 16;
     const int i = i0 + (lane & 15), j = j0 + (lane & 15), t = lane >> 4;
@@ -321,6 +323,44 @@ __global__ __launch_bounds__(64 * W, W == 4 ? 2 : 8) void gemm3m_small_kernel(Sm
         __syncthreads();
     }
     if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = (unsigned int)count;
+}
+
+template <int W>
+__global__ __launch_bounds__(64 * W, W == 4 ? 2 : 8) void gemm3m_small_kernel(SmallArgs g, int tilesM, int tiles)
+{
+    small_tile<W>(g, tilesM, tiles, blockIdx.x);
+}
+
+// Grouped batch: one launch runs the small members of a batch; workgroup b finds its member
+// by binary search over the members' first-workgroup prefix (members are unsliced).
+template <int W>
+__global__ __launch_bounds__(64 * W, W == 4 ? 2 : 8) void gemm3m_small_grouped_kernel(const GroupSmall* items,
+                                                                                    int nitems)
+{
+    const long long b = blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].firstBlock <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int idx = __builtin_amdgcn_readfirstlane(lo);
+    const SmallArgs g = items[idx].a;
+    small_tile<W>(g, items[idx].tilesM, items[idx].tiles, (int)(b - items[idx].firstBlock));
+}
+
+hipError_t launch_gemm_small_grouped(const GroupSmall* items, int nitems, long long blocks, int waves, hipStream_t s)
+{
+    if (nitems <= 0 || blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (waves == 1)
+        hipLaunchKernelGGL(gemm3m_small_grouped_kernel<1>, dim3((unsigned)blocks), dim3(64), 0, s, items, nitems);
+    else if (waves == 4)
+        hipLaunchKernelGGL(gemm3m_small_grouped_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, s, items, nitems);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 // g.arrive must hold zeros when g.slices > 1 (launch_fill_u32 before the first such launch on

@@ -113,6 +113,8 @@ struct GroupedLayout {
     std::vector<Plan> plans;
     std::vector<size_t> base;       // workspace offset of each problem's region
     std::vector<Problem> scaleOnly; // alpha == 0 or k == 0 with beta != 1
+    std::vector<Problem> small;     // members on the one-launch kernel (no pack, no workspace)
+    std::vector<Plan> smallPlans;
     size_t descBytes = 0, offQueue = 0, total = 0;
     size_t nProb = 0, nPack = 0, nTile128 = 0, nTile64 = 0, nMerge = 0;
     long long packBlocks = 0;
@@ -121,26 +123,38 @@ struct GroupedLayout {
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// Plan of one grouped member: the tuned table's (tile, K partitions) for grouped members,
-// else 128 tiles for members of at least 256 x 256 and 64 tiles below, no K split; no
-// rounding correction; the caller's tile/kSplit/identity override when given. The one-problem
-// form of the same plan (variant: LDS-free kernel for 128 tiles, as the grouped kernel
-// uses) gives byte-identical results, which the strided-batched entry relies on.
+// Plan of one grouped member: the tuned table's plan for grouped members (tile 64 / 128 with
+// K partitions, or the one-launch kernel, tile 32 / 16, unsliced), else 128 tiles for members
+// of at least 256 x 256 and 64 tiles below, no K split; no rounding correction; the caller's
+// tile/kSplit/identity override when given (variant 5: the one-launch kernel). The one-problem
+// form of the same plan (variant: LDS-free kernel for 128 tiles, as the grouped kernel uses)
+// gives byte-identical results, which the strided-batched entry relies on.
 aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over)
 {
     aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0, 1};
     int tt = 0, tp = 0;
-    if (over != nullptr && over->tile != 0) {
+    bool small = false;
+    if (over != nullptr && over->variant == 5) {
+        small = true;
+        tt = over->tile == 16 ? 16 : 32;
+    } else if (over != nullptr && over->tile != 0) {
         cfg.tile = over->tile;
         cfg.kSplit = over->kSplit != 0 ? over->kSplit : 1;
     } else if (tuned_lookup(m, n, k, true, &tt, &tp)) {
-        cfg.tile = tt;
-        cfg.kSplit = tp;
+        small = tt == 32 || tt == 16;
+        cfg.tile = small ? 0 : tt;
+        cfg.kSplit = small ? 1 : tp;
     } else {
         cfg.tile = (m >= 256 && n >= 256) ? 128 : 64;
     }
     if (over != nullptr && over->identity != 0) cfg.identity = over->identity;
-    cfg.variant = cfg.tile == 128 ? 3 : 1;
+    if (small) {  // four waves per tile (table tile 32) or one (tile 16); no k slices
+        cfg.variant = 5;
+        cfg.tile = tt == 16 ? 16 : 0;
+        cfg.kSplit = 1;
+    } else {
+        cfg.variant = cfg.tile == 128 ? 3 : 1;
+    }
     return cfg;
 }
 
@@ -173,6 +187,11 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
         const aclblasCgemm3mConfig cfg = grouped_member_config(p.m, p.n, p.k, over);
         Plan pl;
         if (!make_plan(p, &cfg, h->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
+        if (pl.path == 5) {  // the one-launch kernel: no pack, no workspace
+            L.small.push_back(p);
+            L.smallPlans.push_back(pl);
+            continue;
+        }
         L.probs.push_back(p);
         L.plans.push_back(pl);
     }
@@ -191,7 +210,8 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
     L.nProb = L.probs.size();
     L.nPack = 2 * L.nProb;
     L.descBytes = align256(L.nProb * sizeof(GroupProblem)) + align256(L.nPack * sizeof(GroupPack)) +
-                  align256((L.nTile128 + L.nTile64) * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge));
+                  align256((L.nTile128 + L.nTile64) * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge)) +
+                  align256(L.small.size() * sizeof(GroupSmall));
     L.offQueue = total;
     total += 256;
     L.total = L.nProb ? total : 0;
@@ -262,11 +282,14 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
     hipStream_t s = handle->stream;
     hipError_t e = hipSuccess;
-    if (L.nProb > 0) {
+    if (L.nProb > 0 || !L.small.empty()) {
         char* ws = nullptr;
-        st = acquire_workspace(handle, L.total, &ws, op);
-        if (st != ACLBLAS_STATUS_SUCCESS) return st;
+        if (L.nProb > 0) {
+            st = acquire_workspace(handle, L.total, &ws, op);
+            if (st != ACLBLAS_STATUS_SUCCESS) return st;
+        }
         // Descriptor block: [problems][pack items][tiles: 128s then 64s][merge items]
+        // [one-launch members: four-wave form, then single-wave form]
         std::vector<char> blob(L.descBytes, 0);
         GroupProblem* gp = reinterpret_cast<GroupProblem*>(blob.data());
         const size_t offPack = align256(L.nProb * sizeof(GroupProblem));
@@ -377,22 +400,49 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         size_t ti = 0;
         for (auto& c : t128) tl[ti++] = c.t;
         for (auto& c : t64) tl[ti++] = c.t;
+        // one-launch members, grouped by waves per tile (one launch per form)
+        const size_t offSmall = offMerge + align256(L.nMerge * sizeof(GroupMerge));
+        GroupSmall* sm = reinterpret_cast<GroupSmall*>(blob.data() + offSmall);
+        size_t nSmall[2] = {0, 0};     // four-wave, single-wave members
+        long long smallBlocks[2] = {0, 0};
+        for (int form = 0; form < 2; ++form)
+            for (size_t i = 0; i < L.small.size(); ++i) {
+                const Problem& p = L.small[i];
+                const Plan& pl = L.smallPlans[i];
+                if ((pl.smallWaves == 1) != (form == 1)) continue;
+                GroupSmall& it = sm[nSmall[0] + nSmall[1]];
+                it.a = SmallArgs{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.m, p.n, p.k, pl.identity,
+                                 p.alpha.real, p.alpha.imag, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0,
+                                 p.C, p.ldc, 0, 0.0, 0.0, nullptr, 1, pl.smallWaves, nullptr, nullptr, nullptr};
+                it.tilesM = (p.m + 15) / 16;
+                it.tiles = (int)pl.smallTiles;
+                it.firstBlock = smallBlocks[form];
+                smallBlocks[form] += pl.smallTiles;
+                ++nSmall[form];
+            }
 
         const char* dDesc = stage_descriptors(handle, blob, s, &e);
         if (dDesc == nullptr) {
             log_error(op, "descriptor staging failed: %s", hipGetErrorString(e));
             return ACLBLAS_STATUS_ALLOC_FAILED;
         }
-        unsigned int* queue = reinterpret_cast<unsigned int*>(ws + L.offQueue);
+        const GroupSmall* dSmall = reinterpret_cast<const GroupSmall*>(dDesc + offSmall);
+        for (int form = 0; form < 2 && e == hipSuccess; ++form) {
+            if (nSmall[form] == 0) continue;
+            StageTimer t(handle, ST_GEMM);
+            e = launch_gemm_small_grouped(dSmall + (form == 0 ? 0 : nSmall[0]), (int)nSmall[form], smallBlocks[form],
+                                          form == 0 ? 4 : 1, s);
+        }
+        unsigned int* queue = L.nProb > 0 ? reinterpret_cast<unsigned int*>(ws + L.offQueue) : nullptr;
         const GroupProblem* dProb = reinterpret_cast<const GroupProblem*>(dDesc);
         const GroupPack* dPack = reinterpret_cast<const GroupPack*>(dDesc + offPack);
         const GroupTile* dTiles = reinterpret_cast<const GroupTile*>(dDesc + offTiles);
         const GroupMerge* dMerge = reinterpret_cast<const GroupMerge*>(dDesc + offMerge);
-        {
+        if (e == hipSuccess && L.nProb > 0) {
             StageTimer t(handle, ST_PACK);
             e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);
         }
-        if (e == hipSuccess) {
+        if (e == hipSuccess && L.nProb > 0) {
             StageTimer t(handle, ST_GEMM);
             const int cus = handle->cus > 0 ? handle->cus : 256;
             if (!t128.empty())
@@ -444,11 +494,15 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle
                                                           const int* ldc, const aclblasCgemm3mConfig* config)
 {
     const char* op = "aclblasCgemm3mGroupedBatchedEx";
-    if (config != nullptr && ((config->tile != 0 && config->tile != 64 && config->tile != 128) ||
-                              config->kSplit < 0 || config->kSplit > 16 || config->identity < 0 ||
-                              config->identity > ID_STANDARD)) {
-        log_error(op, "invalid grouped config (tile %d, kSplit %d, identity %d)", config->tile, config->kSplit,
-                  config->identity);
+    // variant 5 pins the one-launch kernel for every member (tile 16: its single-wave form;
+    // members are not k-sliced); otherwise tile 64 / 128 and kSplit pin the packed kernels
+    const bool small = config != nullptr && config->variant == 5;
+    if (config != nullptr &&
+        ((small ? (config->tile != 0 && config->tile != 16) || config->kSplit > 1
+                : config->tile != 0 && config->tile != 64 && config->tile != 128) ||
+         config->kSplit < 0 || config->kSplit > 16 || config->identity < 0 || config->identity > ID_STANDARD)) {
+        log_error(op, "invalid grouped config (tile %d, kSplit %d, identity %d, variant %d)", config->tile,
+                  config->kSplit, config->identity, config->variant);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     return grouped_run(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,

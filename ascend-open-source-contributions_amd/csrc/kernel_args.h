@@ -180,5 +180,10 @@ struct GroupPack {        // one operand of one problem for the grouped pack lau
 struct GroupMerge {       // one K-split problem for the grouped merge launch
     MergeArgs a;
 };
+struct GroupSmall {       // one problem of the grouped one-launch (small-problem) kernel
+    SmallArgs a;          // unsliced, no correction
+    int tilesM, tiles;    // 16x16 output tiles: per column of tiles, in all
+    long long firstBlock; // exclusive prefix of workgroups
+};
 
 }  // namespace c3m

@@ -17,6 +17,7 @@ hipError_t launch_gemm_lowp_products(const GemmArgs& g, int pieces, bool lds, hi
 int lowp_products_lds_unroll(int pieces);
 hipError_t launch_gemm_f16(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s);
 hipError_t launch_gemm_small(const SmallArgs& g, hipStream_t s);
+hipError_t launch_gemm_small_grouped(const GroupSmall* items, int nitems, long long blocks, int waves, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
 hipError_t launch_scale_c(const ScaleCArgs& a, hipStream_t s);
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s);

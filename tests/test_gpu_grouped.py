@@ -64,6 +64,32 @@ def test_grouped_vs_oracle(gpu, general):
         assert same, "grouped rerun differs in problem %d" % i
 
 
+@pytest.mark.parametrize("tile", [0, 16])
+def test_grouped_one_launch_members(gpu, tile):
+    """Members on the one-launch kernel (pinned with variant 5; tile 16 is its single-wave
+    form): the grouped call gives the bytes of per-problem calls with the same plan, and
+    meets the oracle."""
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(12, 51 + tile, True)
+    cfg = cgemm3m.config(variant=5, tile=tile, k_split=1, correction=-1)
+    cgemm3m.grouped(gpu, probs, cfg=cfg)
+    torch.cuda.synchronize()
+    grouped = [h["dC"].cpu().numpy().copy() for h in host]
+    check_all(probs, host, "grouped-one-launch")
+    for i, (p, h) in enumerate(zip(probs, host)):
+        h["dC"].copy_(torch.from_numpy(h["C"]).cuda())
+        cgemm3m.cgemm3m(gpu, p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["A"], p["lda"], p["B"],
+                        p["ldb"], p["beta"], p["C"], p["ldc"], cfg)
+        torch.cuda.synchronize()
+        plan = gpu.last_plan()
+        assert plan["path"] == 5, plan
+        same = h["dC"].cpu().numpy().tobytes() == grouped[i].tobytes()
+        assert same, "member %d differs from its one-problem call" % i
+
+
 def test_grouped_edge_problems(gpu):
     """Empty, k = 0 and alpha = 0 members are handled inside one grouped call."""
     import torch
