@@ -426,13 +426,42 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             log_error(op, "descriptor staging failed: %s", hipGetErrorString(e));
             return ACLBLAS_STATUS_ALLOC_FAILED;
         }
+        // The one-launch members share nothing with the packed ones, so when there are both
+        // they run on the handle's second stream, forked after the descriptor upload and joined
+        // before the call returns (under hipGraph capture this becomes two parallel branches).
+        // Their kernel times are not in the stage timings.
         const GroupSmall* dSmall = reinterpret_cast<const GroupSmall*>(dDesc + offSmall);
+        const bool fork = !L.small.empty() && L.nProb > 0;
+        hipStream_t ss = s;
+        if (fork) {
+            if (handle->side == nullptr && hipStreamCreateWithFlags(&handle->side, hipStreamNonBlocking) != hipSuccess)
+                handle->side = nullptr;
+            if (handle->forkEv == nullptr &&
+                hipEventCreateWithFlags(&handle->forkEv, hipEventDisableTiming) != hipSuccess)
+                handle->forkEv = nullptr;
+            if (handle->joinEv == nullptr &&
+                hipEventCreateWithFlags(&handle->joinEv, hipEventDisableTiming) != hipSuccess)
+                handle->joinEv = nullptr;
+            if (handle->side == nullptr || handle->forkEv == nullptr || handle->joinEv == nullptr) {
+                log_error(op, "second stream or its events could not be created");
+                return ACLBLAS_STATUS_EXECUTION_FAILED;
+            }
+            ss = handle->side;
+            e = hipEventRecord(handle->forkEv, s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ss, handle->forkEv, 0);
+        }
         for (int form = 0; form < 2 && e == hipSuccess; ++form) {
             if (nSmall[form] == 0) continue;
-            StageTimer t(handle, ST_GEMM);
-            e = launch_gemm_small_grouped(dSmall + (form == 0 ? 0 : nSmall[0]), (int)nSmall[form], smallBlocks[form],
-                                          form == 0 ? 4 : 1, s);
+            if (fork) {
+                e = launch_gemm_small_grouped(dSmall + (form == 0 ? 0 : nSmall[0]), (int)nSmall[form],
+                                              smallBlocks[form], form == 0 ? 4 : 1, ss);
+            } else {
+                StageTimer t(handle, ST_GEMM);
+                e = launch_gemm_small_grouped(dSmall + (form == 0 ? 0 : nSmall[0]), (int)nSmall[form],
+                                              smallBlocks[form], form == 0 ? 4 : 1, ss);
+            }
         }
+        if (fork && e == hipSuccess) e = hipEventRecord(handle->joinEv, ss);
         unsigned int* queue = L.nProb > 0 ? reinterpret_cast<unsigned int*>(ws + L.offQueue) : nullptr;
         const GroupProblem* dProb = reinterpret_cast<const GroupProblem*>(dDesc);
         const GroupPack* dPack = reinterpret_cast<const GroupPack*>(dDesc + offPack);
@@ -456,6 +485,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             StageTimer t(handle, ST_MERGE);
             e = launch_merge_grouped(dMerge, (int)nm, L.maxMergeOutputs, s);
         }
+        if (fork && e == hipSuccess) e = hipStreamWaitEvent(s, handle->joinEv, 0);
     }
     for (const Problem& p : L.scaleOnly) {
         if (e != hipSuccess) break;

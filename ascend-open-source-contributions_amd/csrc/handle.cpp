@@ -118,6 +118,12 @@ extern "C" aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
             (void)hipEventDestroy(ev.start);
             (void)hipEventDestroy(ev.stop);
         }
+        if (handle->side != nullptr) {
+            if (hipStreamSynchronize(handle->side) != hipSuccess) st = ACLBLAS_STATUS_EXECUTION_FAILED;
+            (void)hipStreamDestroy(handle->side);
+        }
+        if (handle->forkEv != nullptr) (void)hipEventDestroy(handle->forkEv);
+        if (handle->joinEv != nullptr) (void)hipEventDestroy(handle->joinEv);
     }
     handle->userWs = nullptr;
     delete handle;

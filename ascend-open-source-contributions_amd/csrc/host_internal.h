@@ -43,6 +43,10 @@ struct _aclblas_handle {
     };
     std::vector<Staged> staged;
     uint64_t stageClock = 0;
+    // Grouped batches: a second stream (default priority) for the members on the one-launch
+    // kernel, forked from and joined back to `stream` with the two events; created on first use.
+    hipStream_t side = nullptr;
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
 };
 
 namespace c3m {
