@@ -58,8 +58,7 @@ static uint64_t fnv1a(const void* data, size_t bytes)
 static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>& blob, hipStream_t s,
                                      hipError_t* err)
 {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    const bool capturing = hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+    const bool capturing = stream_capturing(s);
     const uint64_t key = fnv1a(blob.data(), blob.size());
     size_t idx = h->staged.size();
     for (size_t i = 0; i < h->staged.size(); ++i) {
@@ -84,6 +83,7 @@ static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>&
             }
         }
         _aclblas_handle::Staged st{key, blob.size(), nullptr, nullptr, false, false, 0};
+        RelaxedCapture relaxed;
         if (hipHostMalloc(&st.host, blob.size(), hipHostMallocDefault) != hipSuccess) {
             *err = hipGetLastError();
             return nullptr;
@@ -434,6 +434,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         const bool fork = !L.small.empty() && L.nProb > 0;
         hipStream_t ss = s;
         if (fork) {
+            RelaxedCapture relaxed;
             if (handle->side == nullptr && hipStreamCreateWithFlags(&handle->side, hipStreamNonBlocking) != hipSuccess)
                 handle->side = nullptr;
             if (handle->forkEv == nullptr &&

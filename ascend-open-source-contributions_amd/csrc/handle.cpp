@@ -52,10 +52,21 @@ aclblasStatus_t ensure_device(aclblasHandle_t h)
     return ACLBLAS_STATUS_SUCCESS;
 }
 
+bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 // The call's temporary storage: the borrowed user workspace when one is set (too small ->
-// EXECUTION_FAILED, api_list.md:203), else the library default, grown on demand.
+// EXECUTION_FAILED, api_list.md:203), else the library default, grown on demand. A default
+// workspace that a capture used is never freed before aclblasDestroy (a graph may replay
+// launches that read it); outgrowing it retires it instead. Growing during a capture
+// retires the old buffer too (no synchronisation is allowed then) and allocates in the
+// relaxed capture mode.
 aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, const char* op)
 {
+    const bool capturing = stream_capturing(h->stream);
     if (h->userWs != nullptr) {
         if (bytes > h->userWsBytes) {
             log_error(op, "user workspace holds %zu bytes, this call needs %zu", h->userWsBytes, bytes);
@@ -67,17 +78,27 @@ aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, 
     const size_t want = std::max(bytes, kDefaultWorkspace);
     if (h->defWs == nullptr || h->defWsBytes < want) {
         if (h->defWs != nullptr) {
-            // Work already queued may still read the old buffer.
-            if (hipStreamSynchronize(h->stream) != hipSuccess) {
-                log_error(op, "stream synchronisation failed before growing the workspace");
-                return ACLBLAS_STATUS_EXECUTION_FAILED;
+            if (capturing || h->defWsCaptured) {
+                h->retiredWs.push_back(h->defWs);
+            } else {
+                // Work already queued may still read the old buffer.
+                if (hipStreamSynchronize(h->stream) != hipSuccess) {
+                    log_error(op, "stream synchronisation failed before growing the workspace");
+                    return ACLBLAS_STATUS_EXECUTION_FAILED;
+                }
+                (void)hipFree(h->defWs);
             }
-            (void)hipFree(h->defWs);
             h->defWs = nullptr;
             h->defWsBytes = 0;
+            h->defWsCaptured = false;
         }
         void* p = nullptr;
-        if (hipMalloc(&p, want) != hipSuccess) {
+        hipError_t e;
+        {
+            RelaxedCapture relaxed;
+            e = hipMalloc(&p, want);
+        }
+        if (e != hipSuccess) {
             (void)hipGetLastError();
             log_error(op, "workspace allocation of %zu bytes failed", want);
             return ACLBLAS_STATUS_ALLOC_FAILED;
@@ -85,6 +106,7 @@ aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, 
         h->defWs = p;
         h->defWsBytes = want;
     }
+    if (capturing) h->defWsCaptured = true;
     *base = static_cast<char*>(h->defWs);
     return ACLBLAS_STATUS_SUCCESS;
 }
@@ -110,6 +132,7 @@ extern "C" aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
     if (handle->device >= 0) {
         if (hipStreamSynchronize(handle->stream) != hipSuccess) st = ACLBLAS_STATUS_EXECUTION_FAILED;
         if (handle->defWs != nullptr) (void)hipFree(handle->defWs);
+        for (void* w : handle->retiredWs) (void)hipFree(w);
         for (auto& sb : handle->staged) {
             (void)hipHostFree(sb.host);
             (void)hipFree(sb.device);

@@ -13,6 +13,9 @@ struct _aclblas_handle {
     hipStream_t stream = nullptr;
     void* defWs = nullptr;      // library-owned default workspace (grow-only)
     size_t defWsBytes = 0;
+    bool defWsCaptured = false;          // a stream capture recorded launches that use defWs
+    std::vector<void*> retiredWs;        // outgrown default workspaces a graph may still use
+                                         // (freed by aclblasDestroy)
     void* userWs = nullptr;     // borrowed via aclblasSetWorkspace
     size_t userWsBytes = 0;
     int device = -1;            // device the default workspace lives on
@@ -57,6 +60,23 @@ constexpr int kVersion = 10000;
 constexpr size_t kMaxStaged = 32;  // descriptor blocks kept per handle besides captured ones
 
 void log_error(const char* op, const char* fmt, ...);
+
+// True when `s` is being captured into a graph.
+bool stream_capturing(hipStream_t s);
+
+// Switches this thread to the relaxed stream-capture mode for its lifetime, so that
+// allocations and stream or event creation made while a caller captures are allowed (a
+// capture in the global mode rejects them and is invalidated).
+class RelaxedCapture {
+  public:
+    RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&mode_); }
+    ~RelaxedCapture() { (void)hipThreadExchangeStreamCaptureMode(&mode_); }
+    RelaxedCapture(const RelaxedCapture&) = delete;
+    RelaxedCapture& operator=(const RelaxedCapture&) = delete;
+
+  private:
+    hipStreamCaptureMode mode_ = hipStreamCaptureModeRelaxed;
+};
 
 enum Stage { ST_PACK = 0, ST_ENERGY = 1, ST_GEMM = 2, ST_MERGE = 3, ST_RECOMPUTE = 4, ST_SCALE = 5, ST_COUNT = 6 };
 

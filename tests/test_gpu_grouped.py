@@ -184,6 +184,77 @@ def test_grouped_hipgraph_replay(gpu):
     assert not failures, "%d problem replays differ" % len(failures)
 
 
+@pytest.mark.parametrize("user_ws", [True, False])
+def test_grouped_first_call_captured(gpu, user_ws):
+    """The first grouped call on a fresh handle is itself captured, with the caller's
+    workspace or the library's default one (allocated during the capture). The batch holds
+    one-launch members (32 x 32) and packed 128-tile members (512 x 512), so the capture
+    forks the second stream; every replay matches an eager call on another handle byte for
+    byte. With the default workspace, an eager call that outgrows it runs between two
+    replays: the captured buffer must stay valid."""
+    import ctypes
+
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(10, 61)
+    for i, p in enumerate(probs):
+        big = i % 2 == 0
+        ta, tb = p["ta"], p["tb"]
+        m = n = 512 if big else 32
+        k = 1024 if big else 128
+        lda, ldb, ldc = (m if ta == "N" else k), (k if tb == "N" else n), m
+        A, B, C = inputs(ta, tb, m, n, k, lda, ldb, ldc, 9000 + 3 * i)
+        dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+        probs[i] = dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=1.0, A=ptr(dA), lda=lda, B=ptr(dB), ldb=ldb, beta=0.0,
+                        C=ptr(dC), ldc=ldc)
+        host[i] = dict(A=A, B=B, C=C, dA=dA, dB=dB, dC=dC)
+    ref = cgemm3m.Handle()
+    cgemm3m.grouped(ref, probs)
+    torch.cuda.synchronize()
+    eager = [x["dC"].cpu().numpy().copy() for x in host]
+    ref.close()
+    check_all(probs, host, "eager")
+    for x in host:
+        x["dC"].copy_(torch.from_numpy(x["C"]).cuda())
+    torch.cuda.synchronize()
+    h = cgemm3m.Handle()
+    need = cgemm3m.grouped_workspace_size(h, probs)
+    ws = torch.empty(max(need, 256), dtype=torch.uint8, device="cuda")
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        h.set_stream(torch.cuda.current_stream())  # (resets the workspace to the default)
+        if user_ws:
+            st = h.L.aclblasSetWorkspace(h.h, ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws.numel()))
+            assert st == 0
+        cgemm3m.grouped(h, probs)
+    torch.cuda.synchronize()
+    for rep in range(2):
+        if rep == 1 and not user_ws:
+            # 4096^3 on the packed path needs far more than the default workspace holds
+            big = 4096
+            X = torch.zeros(big * big, dtype=torch.complex64, device="cuda")
+            Y = torch.zeros(big * big, dtype=torch.complex64, device="cuda")
+            Z = torch.zeros(big * big, dtype=torch.complex64, device="cuda")
+            h.set_stream(torch.cuda.current_stream())
+            cgemm3m.cgemm3m(h, "N", "N", big, big, big, 1.0, ptr(X), big, ptr(Y), big, 0.0, ptr(Z), big)
+            torch.cuda.synchronize()
+            assert h.last_plan()["workspace"] > need
+            del X, Y, Z
+        for x in host:
+            x["dC"].copy_(torch.from_numpy(x["C"]).cuda())
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        bad = [i for i, (x, e) in enumerate(zip(host, eager)) if x["dC"].cpu().numpy().tobytes() != e.tobytes()]
+        print("replay", rep, "differing problems:", bad)
+        sys.stdout.flush()
+        assert not bad
+    del graph
+    h.close()
+
+
 def test_single_call_hipgraph_replay(gpu):
     """aclblasCgemm3m (FP32, fused path with correction) captured once, replayed 3 times."""
     import torch
