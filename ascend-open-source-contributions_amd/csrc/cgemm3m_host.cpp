@@ -147,6 +147,14 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.path = is_one(p.beta) ? 0 : 3;
         return true;
     }
+    // Padded extents and the kernels' element and chunk counts are 32-bit integers: every
+    // dimension must stay below INT_MAX after padding (m, n to 256; k to the deepest stage
+    // group).
+    constexpr int kMaxDim = std::numeric_limits<int>::max() - 4096;
+    if (p.m > kMaxDim || p.n > kMaxDim || p.k > kMaxDim) {
+        log_error(op, "m, n and k above %d are not supported", kMaxDim);
+        return false;
+    }
     switch (pl.precision) {
         case ACLBLAS_CGEMM3M_FP32: pl.format = FMT_F32; pl.pieces = 1; pl.bk = 16; break;
         case ACLBLAS_CGEMM3M_BF16X3: pl.format = FMT_BF16; pl.pieces = 2; pl.bk = 16; break;

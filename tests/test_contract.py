@@ -135,6 +135,15 @@ def test_workspace_size_query(h):
     bf = cgemm3m.workspace_size(h, "N", "N", 8192, 8192, 8192, 1, 0, cgemm3m.config("bf16x3"))
     assert bf >= 6 * 8192 * 8192 * 4
     assert cgemm3m.workspace_size(h, "N", "N", 0, 5, 5, 1, 0) == 0
+    # padded extents and kernel counters are 32-bit: dimensions near INT_MAX are refused
+    # with INVALID_VALUE instead of overflowing (an m x 1 x 1 product with m = 2^31 - 1 is
+    # a valid call whose operands fit in HBM)
+    top = 2**31 - 1
+    for dims in ((top, 1, 1), (1, top, 1), (1, 1, top), (64, 64, top - 100)):
+        with pytest.raises(cgemm3m.Cgemm3mError) as err:
+            cgemm3m.workspace_size(h, "N", "N", *dims, 1, 0)
+        assert err.value.status == _lib.INVALID_VALUE, dims
+    assert cgemm3m.workspace_size(h, "N", "N", 1, 1, top - 4096, 1, 0) >= 0
 
 
 def test_header_is_plain_c99(tmp_path):
