@@ -37,7 +37,7 @@ struct F32Tile {
 };
 
 // One output tile (tmI, tnI) over K partition `part`, start to finish; lds holds
-// 2 * F32Tile<BM, BN, WM, WN>::S4 float4. Used by the one-problem kernel and the grouped kernel.
+// 2 * F32Tile<BM, BN, WM, WN>::S4 float4. Used by the one-problem LDS kernels.
 template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2>
 __device__ __forceinline__ void f32_tile(const GemmArgs& g, int part, int tmI, int tnI, f32x4* lds)
 {
@@ -181,7 +181,8 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void gemm3m_f32_kernel(GemmArgs g)
 // per k slot; the epilogue stores them as (re, im).
 //
 // KFD: 8-deep k fragments per register set (stage); NSET: register sets in the ring, so the
-// loads of stage s + NSET - 1 are issued before the MFMAs of stage s.
+// loads of stage s + NSET - 1 are issued before the MFMAs of stage s. SOLO: every wave of the
+// workgroup computes a tile of its own (WM = WN = 1; the caller gives each wave its tile).
 #ifndef C3M_DIRECT_KFD
 #define C3M_DIRECT_KFD 2
 #endif
@@ -195,16 +196,17 @@ struct DirectSet {
 };
 
 template <int BM, int BN, bool PARTIAL, int WM = 2, int WN = 2, bool FOUR = false, int KFD = C3M_DIRECT_KFD,
-          int NSET = C3M_DIRECT_NSET>
+          int NSET = C3M_DIRECT_NSET, bool SOLO = false>
 __device__ __forceinline__ void f32_tile_direct(const GemmArgs& g, int part, int tmI, int tnI)
 {
+    static_assert(!SOLO || (WM == 1 && WN == 1), "a solo wave owns its whole tile");
     using T = F32Tile<BM, BN, WM, WN>;
     constexpr int NB = FOUR ? 2 : 3;    // B planes
     constexpr int NACC = FOUR ? 2 : 3;  // accumulators per 32x32 output tile
     static_assert(KFD == 1 || KFD == 2, "a planner stage (16 k) is two 8-deep fragments");
     using Set = DirectSet<BM, BN, WM, WN, NB, KFD>;
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = SOLO ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wm = wave % WM, wn = wave / WM;
     // this partition's k fragments [kf0, kf0 + KFD * nst)
     const int kf0 = 2 * (int)((long long)g.kSteps * part / g.parts);
@@ -314,74 +316,44 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm3m_f32_direct_kernel(Ge
     f32_tile_direct<BM, BN, PARTIAL, WM, WN, FOUR>(g, part, tmI, tnI);
 }
 
-// Grouped batch: a persistent grid pulls tiles of many independent problems from one
-// queue (tiles pre-sorted by decreasing cost, so the longest start first). Each tile is
-// computed start to finish by one workgroup with the same arithmetic as the one-problem
-// kernel, so the output does not depend on which workgroup took it. One launch per tile
-// size, so each size runs at its own occupancy.
-template <int BM, int WN>
-__global__ __launch_bounds__(128 * WN, BM == 128 ? 1 : 2) void gemm3m_f32_grouped_kernel(const GroupProblem* probs,
-                                                                                       const GroupTile* tiles,
-                                                                                       int ntiles, unsigned int* queue)
+// Grouped batch on 64 x 64 wave tiles: a persistent grid of four-wave workgroups pulls quads
+// (four wave tiles, entries 4q .. 4q + 3 of the list) from one queue; wave w computes entry
+// 4q + w with the per-wave code of the 128 x 128 LDS-free kernel (a 64 x 64 output block,
+// same MFMA order per accumulator, so the bytes equal the one-problem kernels'). The quad of
+// a whole 128 x 128 region lists its blocks in that kernel's wave order, so the four waves
+// share operand fragments in the CU's cache as they do there; the blocks at the edges of the
+// members (where a 128 x 128 tile would be mostly padding) are grouped four at a time by
+// cost. An entry with problem < 0 is an empty slot. Quads are sorted by decreasing cost.
+__global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_quad_kernel(const GroupProblem* probs,
+                                                                        const GroupTile* tiles, int nquads,
+                                                                        unsigned int* queue)
 {
-    __shared__ f32x4 lds[2 * F32Tile<BM, BM, 2, WN>::S4];
     __shared__ int next;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (;;) {
         if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
         __syncthreads();
-        const int t = __builtin_amdgcn_readfirstlane(next);
+        const int q = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();
-        if (t >= ntiles) break;
-        const GroupTile gt = tiles[t];
+        if (q >= nquads) break;
+        const GroupTile gt = tiles[(size_t)q * 4 + wave];
         const int p = __builtin_amdgcn_readfirstlane(gt.problem);
+        if (p < 0) continue;
         const GemmArgs g = probs[p].g;
         const int part = __builtin_amdgcn_readfirstlane(gt.part);
         const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
         const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
         if (g.parts > 1)
-            f32_tile<BM, BM, true, 2, WN>(g, part, tm, tn, lds);
+            f32_tile_direct<64, 64, true, 1, 1, false, C3M_DIRECT_KFD, C3M_DIRECT_NSET, true>(g, part, tm, tn);
         else
-            f32_tile<BM, BM, false, 2, WN>(g, part, tm, tn, lds);
-        __syncthreads();
+            f32_tile_direct<64, 64, false, 1, 1, false, C3M_DIRECT_KFD, C3M_DIRECT_NSET, true>(g, part, tm, tn);
     }
 }
 
-// Grouped 128x128 tiles with the LDS-free four-wave tile (the one-problem default).
-__global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_direct_kernel(const GroupProblem* probs,
-                                                                          const GroupTile* tiles, int ntiles,
-                                                                          unsigned int* queue)
+hipError_t launch_gemm_f32_grouped_quads(const GroupProblem* probs, const GroupTile* tiles, int nquads,
+                                         unsigned int* queue, int blocks, hipStream_t s)
 {
-    __shared__ int next;
-    for (;;) {
-        if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
-        __syncthreads();
-        const int t = __builtin_amdgcn_readfirstlane(next);
-        __syncthreads();
-        if (t >= ntiles) break;
-        const GroupTile gt = tiles[t];
-        const int p = __builtin_amdgcn_readfirstlane(gt.problem);
-        const GemmArgs g = probs[p].g;
-        const int part = __builtin_amdgcn_readfirstlane(gt.part);
-        const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
-        const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
-        if (g.parts > 1)
-            f32_tile_direct<128, 128, true, 2, 2>(g, part, tm, tn);
-        else
-            f32_tile_direct<128, 128, false, 2, 2>(g, part, tm, tn);
-    }
-}
-
-hipError_t launch_gemm_f32_grouped(const GroupProblem* probs, const GroupTile* tiles, int ntiles,
-                                   unsigned int* queue, int blocks, int tile, hipStream_t s)
-{
-    // 128 tiles: the LDS-free four-wave tile (as the one-problem default); 64 tiles: four
-    // waves through LDS.
-    if (tile == 128)
-        hipLaunchKernelGGL(gemm3m_f32_grouped_direct_kernel, dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles,
-                           queue);
-    else
-        hipLaunchKernelGGL((gemm3m_f32_grouped_kernel<64, 2>), dim3(blocks), dim3(256), 0, s, probs, tiles, ntiles,
-                           queue);
+    hipLaunchKernelGGL(gemm3m_f32_grouped_quad_kernel, dim3(blocks), dim3(256), 0, s, probs, tiles, nquads, queue);
     return hipGetLastError();
 }
 

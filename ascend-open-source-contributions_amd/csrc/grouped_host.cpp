@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "host_internal.h"
@@ -116,7 +117,7 @@ struct GroupedLayout {
     std::vector<Problem> small;     // members on the one-launch kernel (no pack, no workspace)
     std::vector<Plan> smallPlans;
     size_t descBytes = 0, offQueue = 0, total = 0;
-    size_t nProb = 0, nPack = 0, nTile128 = 0, nTile64 = 0, nMerge = 0;
+    size_t nProb = 0, nPack = 0, nQuad = 0, nMerge = 0;
     long long packBlocks = 0;
     int maxMergeOutputs = 0;
 };
@@ -196,21 +197,25 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
         L.plans.push_back(pl);
     }
     size_t total = 0;
+    size_t loose = 0;  // 64 x 64 blocks outside whole 128 x 128 regions
     for (size_t i = 0; i < L.probs.size(); ++i) {
         L.base.push_back(total);
         total = align256(total + L.plans[i].total);
         const Plan& pl = L.plans[i];
-        const long long tiles = (long long)((L.probs[i].m + pl.bm - 1) / pl.bm) * ((L.probs[i].n + pl.bn - 1) / pl.bn);
-        (pl.bm == 128 ? L.nTile128 : L.nTile64) += (size_t)(tiles * pl.parts);
+        const size_t bm = (size_t)(L.probs[i].m + 63) / 64, bn = (size_t)(L.probs[i].n + 63) / 64;
+        const size_t whole = (bm / 2) * (bn / 2);
+        L.nQuad += whole * (size_t)pl.parts;
+        loose += (bm * bn - 4 * whole) * (size_t)pl.parts;
         if (pl.parts > 1) {
             ++L.nMerge;
             L.maxMergeOutputs = std::max<long long>(L.maxMergeOutputs, (long long)L.probs[i].m * L.probs[i].n);
         }
     }
+    L.nQuad += (loose + 3) / 4;
     L.nProb = L.probs.size();
     L.nPack = 2 * L.nProb;
     L.descBytes = align256(L.nProb * sizeof(GroupProblem)) + align256(L.nPack * sizeof(GroupPack)) +
-                  align256((L.nTile128 + L.nTile64) * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge)) +
+                  align256(4 * L.nQuad * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge)) +
                   align256(L.small.size() * sizeof(GroupSmall));
     L.offQueue = total;
     total += 256;
@@ -288,7 +293,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             st = acquire_workspace(handle, L.total, &ws, op);
             if (st != ACLBLAS_STATUS_SUCCESS) return st;
         }
-        // Descriptor block: [problems][pack items][tiles: 128s then 64s][merge items]
+        // Descriptor block: [problems][pack items][quads: four 64 x 64 blocks each][merge items]
         // [one-launch members: four-wave form, then single-wave form]
         std::vector<char> blob(L.descBytes, 0);
         GroupProblem* gp = reinterpret_cast<GroupProblem*>(blob.data());
@@ -296,13 +301,19 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         GroupPack* pk = reinterpret_cast<GroupPack*>(blob.data() + offPack);
         const size_t offTiles = offPack + align256(L.nPack * sizeof(GroupPack));
         GroupTile* tl = reinterpret_cast<GroupTile*>(blob.data() + offTiles);
-        const size_t offMerge = offTiles + align256((L.nTile128 + L.nTile64) * sizeof(GroupTile));
+        const size_t offMerge = offTiles + align256(4 * L.nQuad * sizeof(GroupTile));
         GroupMerge* mg = reinterpret_cast<GroupMerge*>(blob.data() + offMerge);
         struct Cost {
             long long cost;
             GroupTile t;
         };
-        std::vector<Cost> t128, t64;
+        struct Quad {
+            long long cost;
+            GroupTile t[4];
+        };
+        std::vector<Quad> quads;
+        std::vector<Cost> loose;
+        quads.reserve(L.nQuad);
         long long packBlocks = 0;
         size_t nm = 0;
         for (size_t i = 0; i < L.nProb; ++i) {
@@ -360,14 +371,24 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             g.ldc = p.ldc;
             g.partial = pl.parts > 1 ? reinterpret_cast<float*>(pws + pl.offPartial) : nullptr;
             g.partStride = pl.partStride;
-            gp[i].tile = pl.bm;
+            // 64 x 64 blocks: whole 128 x 128 regions as quads in the 128 x 128 kernel's wave
+            // order (wave w: block row 2r + w % 2, block column 2c + w / 2); the odd last block
+            // row and column go to the loose list
             const long long stepsPerPart = (pl.kSteps + pl.parts - 1) / pl.parts;
-            for (int part = 0; part < pl.parts; ++part)
-                for (int tn = 0; tn < g.tilesN; ++tn)
-                    for (int tm = 0; tm < g.tilesM; ++tm) {
-                        Cost c{stepsPerPart * pl.bm * pl.bn, GroupTile{(int)i, part, tm, tn}};
-                        (pl.bm == 128 ? t128 : t64).push_back(c);
+            const int bm = (p.m + 63) / 64, bn = (p.n + 63) / 64;
+            for (int part = 0; part < pl.parts; ++part) {
+                for (int tn = 0; tn + 1 < bn; tn += 2)
+                    for (int tm = 0; tm + 1 < bm; tm += 2) {
+                        Quad qd;
+                        qd.cost = stepsPerPart;
+                        for (int w = 0; w < 4; ++w) qd.t[w] = GroupTile{(int)i, part, tm + w % 2, tn + w / 2};
+                        quads.push_back(qd);
                     }
+                for (int tn = 0; tn < bn; ++tn)
+                    for (int tm = 0; tm < bm; ++tm)
+                        if ((bm % 2 == 1 && tm == bm - 1) || (bn % 2 == 1 && tn == bn - 1))
+                            loose.push_back(Cost{stepsPerPart, GroupTile{(int)i, part, tm, tn}});
+            }
             if (pl.parts > 1) {
                 MergeArgs& a = mg[nm++].a;
                 a = MergeArgs{};
@@ -386,20 +407,34 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 a.ldc = p.ldc;
             }
         }
-        // Longest tiles first; ties in (problem, part, tm, tn) order, so the list is a pure
-        // function of the arguments.
-        auto byCost = [](const Cost& a, const Cost& b) {
-            if (a.cost != b.cost) return a.cost > b.cost;
-            if (a.t.problem != b.t.problem) return a.t.problem < b.t.problem;
-            if (a.t.part != b.t.part) return a.t.part < b.t.part;
-            if (a.t.tn != b.t.tn) return a.t.tn < b.t.tn;
-            return a.t.tm < b.t.tm;
+        // Longest first; ties in (problem, part, tn, tm) order, so the list is a pure function
+        // of the arguments. Loose blocks are grouped four at a time in that order (blocks of
+        // like cost share a workgroup); the last quad is filled with empty slots.
+        auto before = [](long long ca, const GroupTile& a, long long cb, const GroupTile& b) {
+            if (ca != cb) return ca > cb;
+            if (a.problem != b.problem) return a.problem < b.problem;
+            if (a.part != b.part) return a.part < b.part;
+            if (a.tn != b.tn) return a.tn < b.tn;
+            return a.tm < b.tm;
         };
-        std::sort(t128.begin(), t128.end(), byCost);
-        std::sort(t64.begin(), t64.end(), byCost);
+        std::sort(loose.begin(), loose.end(),
+                  [&](const Cost& a, const Cost& b) { return before(a.cost, a.t, b.cost, b.t); });
+        for (size_t j = 0; j < loose.size(); j += 4) {
+            Quad qd;
+            qd.cost = loose[j].cost;
+            for (size_t w = 0; w < 4; ++w)
+                qd.t[w] = j + w < loose.size() ? loose[j + w].t : GroupTile{-1, 0, 0, 0};
+            quads.push_back(qd);
+        }
+        std::sort(quads.begin(), quads.end(),
+                  [&](const Quad& a, const Quad& b) { return before(a.cost, a.t[0], b.cost, b.t[0]); });
+        if (quads.size() != L.nQuad || L.nQuad > (size_t)std::numeric_limits<int>::max()) {
+            log_error(op, "internal error: %zu quads laid out, %zu counted", quads.size(), L.nQuad);
+            return ACLBLAS_STATUS_INTERNAL_ERROR;
+        }
         size_t ti = 0;
-        for (auto& c : t128) tl[ti++] = c.t;
-        for (auto& c : t64) tl[ti++] = c.t;
+        for (const Quad& qd : quads)
+            for (int w = 0; w < 4; ++w) tl[ti++] = qd.t[w];
         // one-launch members, grouped by waves per tile (one launch per form)
         const size_t offSmall = offMerge + align256(L.nMerge * sizeof(GroupMerge));
         GroupSmall* sm = reinterpret_cast<GroupSmall*>(blob.data() + offSmall);
@@ -475,12 +510,9 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         if (e == hipSuccess && L.nProb > 0) {
             StageTimer t(handle, ST_GEMM);
             const int cus = handle->cus > 0 ? handle->cus : 256;
-            if (!t128.empty())
-                e = launch_gemm_f32_grouped(dProb, dTiles, (int)t128.size(), queue, std::min<int>((int)t128.size(), cus),
-                                            128, s);
-            if (e == hipSuccess && !t64.empty())
-                e = launch_gemm_f32_grouped(dProb, dTiles + t128.size(), (int)t64.size(), queue + 1,
-                                            std::min<int>((int)t64.size(), 2 * cus), 64, s);
+            if (!quads.empty())
+                e = launch_gemm_f32_grouped_quads(dProb, dTiles, (int)quads.size(), queue,
+                                                  std::min<int>((int)quads.size(), cus), s);
         }
         if (e == hipSuccess && nm > 0) {
             StageTimer t(handle, ST_MERGE);

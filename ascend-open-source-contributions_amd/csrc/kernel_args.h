@@ -167,8 +167,7 @@ struct ResidualArgs {   // FP64 reference GEMM compared with the FP32 result
 // Grouped batch descriptors (device arrays in the workspace).
 struct GroupProblem {
     GemmArgs g;           // complete arguments of this problem's product kernel
-    int tile;             // 64 or 128
-    int pad_[3];
+    int pad_[4];
 };
 struct GroupTile {        // one output tile of one problem (and one K partition)
     int problem, part, tm, tn;

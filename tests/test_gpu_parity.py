@@ -187,6 +187,28 @@ def test_kernel_variants_identical(gpu, prec, variants, parts):
             v, variants[0], int(np.count_nonzero(o.view(np.uint64) != outs[0].view(np.uint64))))
 
 
+@pytest.mark.parametrize("parts", [1, 3])
+def test_tile_sizes_identical(gpu, parts):
+    """64 x 64 tiles through LDS (four waves of 32 x 32) and 128 x 128 tiles loading
+    fragments straight into registers (four waves of 64 x 64) issue the same MFMAs per
+    accumulator in the same k order, so the output bytes do not depend on the tile size (the
+    grouped batch relies on it: it computes every member on 64 x 64 wave tiles)."""
+    import cgemm3m
+
+    m, n, k = 300, 200, 700
+    A, B, C = inputs("T", "C", m, n, k, k + 1, n + 2, m + 3, 20260580 + parts)
+    outs = []
+    for tile, variant in ((64, 1), (128, 3), (128, 1)):
+        cfg = cgemm3m.config(tile=tile, k_split=parts, variant=variant)
+        st, out = run_gpu(gpu, "T", "C", m, n, k, ALPHA2, A, k + 1, B, n + 2, BETA2, C, m + 3, cfg)
+        assert st == 0
+        outs.append(out)
+    for i in (1, 2):
+        same = outs[i].tobytes() == outs[0].tobytes()
+        assert same, "form %d differs from the 64 x 64 tile in %d outputs" % (
+            i, int(np.count_nonzero(outs[i].view(np.uint64) != outs[0].view(np.uint64))))
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_small_ragged(gpu, seed):
     rng = np.random.default_rng(seed)
