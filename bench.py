@@ -49,6 +49,9 @@ def main():
                     help="for rocprofv3 counter passes: 3 NN calls of the first --variants form (fp32 "
                          "by default), then one device copy of exactly 1 GiB (calibrates FETCH_SIZE/"
                          "WRITE_SIZE), nothing timed")
+    ap.add_argument("--grouped-only", action="store_true",
+                    help="for kernel traces: only the 512-problem grouped batch (one call, its hipGraph "
+                         "replay, the per-problem loop); prints its figures as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path here (across GPUs the "
@@ -62,6 +65,11 @@ def main():
 
     m = n = k = args.size
     dev = torch.device("cuda:0")
+    if args.grouped_only:
+        hg = cgemm3m.Handle(stream=torch.cuda.current_stream())
+        print(json.dumps({"grouped_512": grouped_bench(hg, torch, cgemm3m, dev)}), flush=True)
+        hg.close()
+        return
     gen = torch.Generator(device=dev)
     gen.manual_seed(20261203)
 
@@ -292,7 +300,9 @@ def autograd_bench(torch, cgemm3m, dev, args, m=2048, k=1024, n=4096, seed=777):
 def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
     """512 uneven problems (m, n in 16*{2..32}, k in 64*{1..32}, ops uniform over {N,T,C}^2,
     alpha = 1, beta = 0): one grouped call, the same call replayed from a hipGraph, and a
-    loop of one aclblasCgemm3m call per problem. Complex TFLOP/s counts sum(8 m n k)."""
+    loop of one aclblasCgemm3m call per problem, both as the plain call (which turns the
+    rounding-correction screen on for k >= 1024) and with the correction off, as grouped
+    members run. Complex TFLOP/s counts sum(8 m n k)."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -324,13 +334,15 @@ def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps * 1e3
 
-    def loop():
+    def loop(cfg=None):
         for p in probs:
             cgemm3m.cgemm3m(h, p["ta"], p["tb"], p["m"], p["n"], p["k"], 1.0, p["A"], p["lda"], p["B"], p["ldb"],
-                            0.0, p["C"], p["ldc"])
+                            0.0, p["C"], p["ldc"], cfg)
 
     t_group = timed(lambda: cgemm3m.grouped(h, probs))
     t_loop = timed(loop)
+    no_corr = cgemm3m.config(correction=-1)
+    t_loop_nc = timed(lambda: loop(no_corr))
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
@@ -343,9 +355,10 @@ def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
     h.set_stream(stream)
     t_graph = timed(graph.replay)
     return {"problems": count, "complex_tflop": flops / 1e12, "ms_grouped": t_group, "ms_graph_replay": t_graph,
-            "ms_per_problem_loop": t_loop, "tflops_grouped": flops / (t_group * 1e-3) / 1e12,
+            "ms_per_problem_loop": t_loop, "ms_per_problem_loop_no_correction": t_loop_nc,
+            "tflops_grouped": flops / (t_group * 1e-3) / 1e12,
             "tflops_graph": flops / (t_graph * 1e-3) / 1e12, "tflops_loop": flops / (t_loop * 1e-3) / 1e12,
-            "speedup_graph_vs_loop": t_loop / t_graph}
+            "speedup_graph_vs_loop": t_loop / t_graph, "speedup_graph_vs_loop_no_correction": t_loop_nc / t_graph}
 
 
 def cpu_baseline(A, B, C0, m, n, k, alpha, beta, seconds):
