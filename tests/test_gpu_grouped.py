@@ -90,6 +90,34 @@ def test_grouped_one_launch_members(gpu, tile):
         assert same, "member %d differs from its one-problem call" % i
 
 
+@pytest.mark.parametrize("tile,parts", [(128, 3), (64, 2)])
+def test_grouped_k_split_members(gpu, tile, parts):
+    """Members pinned to a K split: the quad launch writes each partition's planes and the
+    grouped merge combines them in partition order; every member equals its one-problem call
+    with the same plan byte for byte and meets the oracle."""
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(10, 71 + tile + parts, True)
+    cfg = cgemm3m.config(tile=tile, k_split=parts, correction=-1)
+    cgemm3m.grouped(gpu, probs, cfg=cfg)
+    torch.cuda.synchronize()
+    grouped = [h["dC"].cpu().numpy().copy() for h in host]
+    check_all(probs, host, "grouped-k-split")
+    one = cgemm3m.config(tile=tile, k_split=parts, correction=-1, variant=3 if tile == 128 else 1)
+    for i, (p, h) in enumerate(zip(probs, host)):
+        h["dC"].copy_(torch.from_numpy(h["C"]).cuda())
+        cgemm3m.cgemm3m(gpu, p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["A"], p["lda"], p["B"],
+                        p["ldb"], p["beta"], p["C"], p["ldc"], one)
+        torch.cuda.synchronize()
+        plan = gpu.last_plan()
+        assert plan["parts"] == min(parts, (p["k"] + 31) // 32 * 32 // 16), plan
+        same = h["dC"].cpu().numpy().tobytes() == grouped[i].tobytes()
+        assert same, "member %d (%s%s %d x %d x %d) differs from its one-problem call" % (
+            i, p["ta"], p["tb"], p["m"], p["n"], p["k"])
+
+
 def test_grouped_edge_problems(gpu):
     """Empty, k = 0 and alpha = 0 members are handled inside one grouped call."""
     import torch
