@@ -39,6 +39,10 @@ SINGLE_SHAPES = [
     (128, 128, 4096), (2048, 2048, 256), (4096, 4096, 512), (64, 64, 1024),
     (64, 64, 64), (128, 128, 128), (128, 128, 512), (384, 384, 384), (512, 512, 128), (768, 768, 768),
     (1024, 1024, 256), (1024, 1024, 512),
+    # rectangular mid sizes (every order of 1024, 2048, 4096: the products of a 2048 x 1024 by
+    # 1024 x 4096 forward and its two gradient products)
+    (1024, 2048, 4096), (1024, 4096, 2048), (2048, 1024, 4096), (2048, 4096, 1024), (4096, 1024, 2048),
+    (4096, 2048, 1024),
 ]
 # square members (even log2 m*n classes) and m x 2m members (the odd classes between them)
 GROUPED_SHAPES = [(m, m * r, k) for m in (32, 64, 128, 256, 512) for r in (1, 2) if m * r <= 512
