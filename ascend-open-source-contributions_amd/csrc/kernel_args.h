@@ -136,6 +136,7 @@ struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/b
     int correction;
     const double* eA; const double* eB; double corrScale, corrTau;
     unsigned long long* bitmap; void* shadow;
+    int cWide;          // set by launch_merge: C is 16-byte aligned and ldc is even
 };
 
 struct ScaleCArgs {     // alpha == 0 or k == 0: C = beta*C, or zeros when beta == 0
