@@ -167,7 +167,9 @@ aclblasStatus_t aclblasCgemm3mGetWorkspaceSize(aclblasHandle_t handle, aclblasOp
                                                const aclblasCgemm3mConfig* config, size_t* bytes);
 
 /* For callers with many small independent problems: `count` problems in one persistent
- * launch (plus one packing launch and, if any problem is K-split, one merge launch).
+ * launch (plus one packing launch and, if any problem is K-split, one merge launch);
+ * problems the autotuned table gives to the one-launch kernel run in one more launch per
+ * form, on a second stream of the handle joined back before the call returns.
  * All arrays are HOST arrays of length count; A/B/C entries are device pointers.
  * Per-problem validation follows aclblasCgemm3m; the first invalid problem's status is
  * returned and nothing is launched. precision is an aclblasCgemm3mPrecision_t. */
