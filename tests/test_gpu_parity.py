@@ -6,6 +6,7 @@ and (c) byte-identical output on a second run (cgemm3m_test.cpp:304-307).
 """
 import itertools
 import sys
+import zlib
 
 import numpy as np
 import pytest
@@ -207,6 +208,97 @@ def test_tile_sizes_identical(gpu, parts):
         same = outs[i].tobytes() == outs[0].tobytes()
         assert same, "form %d differs from the 64 x 64 tile in %d outputs" % (
             i, int(np.count_nonzero(outs[i].view(np.uint64) != outs[0].view(np.uint64))))
+
+
+def _no_neg_zero(x):
+    """The bytes of x with -0.0 read as +0.0: an exactly cancelling sum may carry either sign
+    of zero depending on its order, which the comparison leaves open."""
+    return np.asarray(x).view(np.float32) + np.float32(0.0)
+
+
+def _int_fill(rows, cols, ld, rng, lim=8):
+    """Column-major complex64 storage with integer real and imaginary parts in [-lim, lim]."""
+    x = rng.integers(-lim, lim + 1, size=(2 * max(ld, 1) * max(cols, 0),)).astype(np.float32)
+    return x.view(np.complex64).copy()
+
+
+INT_CASES = [
+    # (label, m, n, k, ops, config kwargs)
+    ("one-launch", 100, 70, 200, "NN", dict()),
+    ("one-launch-1wave", 90, 33, 300, "CT", dict(variant=5, tile=16, k_split=1)),
+    ("one-launch-sliced", 40, 40, 512, "TN", dict(variant=5, k_split=4)),
+    ("tile64-lds", 300, 200, 333, "NC", dict(tile=64, k_split=1)),
+    ("tile128-lds", 300, 200, 333, "TT", dict(tile=128, k_split=1, variant=1)),
+    ("tile128-direct", 300, 200, 333, "CN", dict(tile=128, k_split=1, variant=3)),
+    ("k-split3", 260, 130, 512, "NN", dict(tile=128, k_split=3)),
+    ("four-products", 300, 200, 333, "CC", dict(identity=3)),
+    ("standard-identity", 300, 200, 333, "NT", dict(identity=2, tile=64)),
+    ("bf16x3", 300, 260, 512, "NN", dict(precision="bf16x3")),
+    ("bf16x6", 300, 260, 512, "CT", dict(precision="bf16x6")),
+    ("fp16", 300, 260, 512, "TC", dict(precision="fp16")),
+    ("bf16x3-products", 2048, 1024, 512, "NN", dict(precision="bf16x3", variant=4)),
+    ("fp16-products", 2048, 1024, 512, "CN", dict(precision="fp16", variant=4)),
+]
+
+
+@pytest.mark.parametrize("case", INT_CASES, ids=[c[0] for c in INT_CASES])
+def test_integer_inputs_bitwise(gpu, case):
+    """Integer real and imaginary parts in [-8, 8] with |partial sums| far below 2^24: every
+    product and sum of every path is exact in FP32 (the bf16 hi/lo split leaves lo = 0, the
+    FP16 block scales are powers of two), and so is the reference BLAS's order, so each path
+    must equal the oracle bit for bit. alpha and beta are small Gaussian integers."""
+    import cgemm3m
+
+    label, m, n, k, ops, kw = case
+    ta, tb = ops[0], ops[1]
+    rng = np.random.default_rng(zlib.crc32(label.encode()))
+    lda = (m if ta == "N" else k) + 1
+    ldb = (k if tb == "N" else n) + 2
+    ldc = m + 3
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    A, B, C = _int_fill(ar, ac, lda, rng), _int_fill(br, bc, ldb, rng), _int_fill(m, n, ldc, rng)
+    alpha, beta = complex(2, -1), complex(1, 1)
+    prec = kw.pop("precision", "fp32")
+    cfg = cgemm3m.config(prec, **kw)
+    print("case", label, ops, m, n, k, prec, kw)
+    sys.stdout.flush()
+    st, out = run_gpu(gpu, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, cfg)
+    assert st == 0
+    gold = oracle.cgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    same = _no_neg_zero(out).tobytes() == _no_neg_zero(gold).tobytes()
+    assert same, "%s: %d outputs differ from the oracle" % (
+        label, int(np.count_nonzero(out.view(np.uint64) != gold.view(np.uint64))))
+
+
+def test_integer_inputs_bitwise_grouped(gpu):
+    """The same exactness through one grouped call (packed quads, one-launch members, a K
+    split member)."""
+    import torch
+
+    import cgemm3m
+
+    rng = np.random.default_rng(99)
+    shapes = [(300, 200, 333), (40, 48, 512), (512, 512, 256), (100, 30, 64), (256, 256, 1024)]
+    probs, keep, gold = [], [], []
+    for i, (m, n, k) in enumerate(shapes):
+        ta, tb = "NTC"[i % 3], "NTC"[(i + 1) % 3]
+        lda = (m if ta == "N" else k) + 1
+        ldb = (k if tb == "N" else n) + 1
+        A = _int_fill(*((m, k) if ta == "N" else (k, m)), lda, rng)
+        B = _int_fill(*((k, n) if tb == "N" else (n, k)), ldb, rng)
+        C = _int_fill(m, n, m, rng)
+        dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+        keep += [dA, dB, dC]
+        probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=complex(1, 2), A=ptr(dA), lda=lda, B=ptr(dB), ldb=ldb,
+                          beta=complex(-1, 0), C=ptr(dC), ldc=m))
+        gold.append(oracle.cgemm(ta, tb, m, n, k, complex(1, 2), A, lda, B, ldb, complex(-1, 0), C, m))
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    for i, g in enumerate(gold):
+        out = keep[3 * i + 2].cpu().numpy()
+        same = _no_neg_zero(out).tobytes() == _no_neg_zero(g).tobytes()
+        assert same, "member %d %s: %d outputs differ" % (i, shapes[i], int(np.count_nonzero(out != g)))
 
 
 @pytest.mark.parametrize("seed", range(6))
