@@ -400,15 +400,40 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
         unsigned int* slots = pl.correction ? reinterpret_cast<unsigned int*>(ws + pl.offCounter) : nullptr;
         const bool sliced = pl.parts > 1;
+        // Arrival counters must be zero at launch; the kernel leaves them zero. The handle's
+        // pool is cleared once, when created on an eager call; otherwise (more tiles than the
+        // pool holds, or a capture before the pool exists) the workspace's counters are
+        // cleared by a fill launch first.
+        unsigned int* arrive = nullptr;
+        bool clear = false;
+        if (sliced) {
+            if (pl.smallTiles <= kArrivePool && h->arrivePool == nullptr && !stream_capturing(s)) {
+                void* pool = nullptr;
+                if (hipMalloc(&pool, (size_t)kArrivePool * sizeof(unsigned int)) == hipSuccess) {
+                    e = launch_fill_u64(static_cast<unsigned long long*>(pool), kArrivePool / 2, s);
+                    if (e == hipSuccess)
+                        h->arrivePool = static_cast<unsigned int*>(pool);
+                    else
+                        (void)hipFree(pool);
+                } else {
+                    (void)hipGetLastError();  // fall back to the workspace's counters
+                }
+            }
+            if (pl.smallTiles <= kArrivePool && h->arrivePool != nullptr) {
+                arrive = h->arrivePool;
+            } else {
+                arrive = reinterpret_cast<unsigned int*>(ws + pl.offArrive);
+                clear = true;
+            }
+        }
         SmallArgs a{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.m, p.n, p.k, pl.identity,
                     p.alpha.real, p.alpha.imag, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc,
                     pl.correction, amag * 0x1p-24, 0x1p-8, slots, pl.parts, pl.smallWaves,
                     sliced ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr,
-                    sliced && pl.correction ? reinterpret_cast<double*>(ws + pl.offEPA) : nullptr,
-                    sliced ? reinterpret_cast<unsigned int*>(ws + pl.offArrive) : nullptr};
+                    sliced && pl.correction ? reinterpret_cast<double*>(ws + pl.offEPA) : nullptr, arrive};
         StageTimer t(h, ST_GEMM);
-        if (sliced)  // the arrival counters start at zero (the kernel leaves them zero)
-            e = launch_fill_u64(reinterpret_cast<unsigned long long*>(ws + pl.offArrive), (pl.smallTiles + 1) / 2, s);
+        if (e == hipSuccess && clear)
+            e = launch_fill_u64(reinterpret_cast<unsigned long long*>(arrive), (pl.smallTiles + 1) / 2, s);
         if (e == hipSuccess) e = launch_gemm_small(a, s);
         if (slots != nullptr) {
             h->lastCounter = slots;

@@ -133,6 +133,7 @@ extern "C" aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
         if (hipStreamSynchronize(handle->stream) != hipSuccess) st = ACLBLAS_STATUS_EXECUTION_FAILED;
         if (handle->defWs != nullptr) (void)hipFree(handle->defWs);
         for (void* w : handle->retiredWs) (void)hipFree(w);
+        if (handle->arrivePool != nullptr) (void)hipFree(handle->arrivePool);
         for (auto& sb : handle->staged) {
             (void)hipHostFree(sb.host);
             (void)hipFree(sb.device);

@@ -50,6 +50,10 @@ struct _aclblas_handle {
     // kernel, forked from and joined back to `stream` with the two events; created on first use.
     hipStream_t side = nullptr;
     hipEvent_t forkEv = nullptr, joinEv = nullptr;
+    // Arrival counters of k-sliced one-launch calls (kArrivePool of them), zeroed once when
+    // created (on an eager call); every launch leaves the counters it used at zero, so later
+    // calls need no clearing launch. Freed by aclblasDestroy.
+    unsigned int* arrivePool = nullptr;
 };
 
 namespace c3m {
@@ -58,6 +62,7 @@ constexpr size_t kDefaultWorkspace = 32u << 20;  // cann_ops_blas.h:34-36
 constexpr size_t kAlign = 256;
 constexpr int kVersion = 10000;
 constexpr size_t kMaxStaged = 32;  // descriptor blocks kept per handle besides captured ones
+constexpr long long kArrivePool = 1 << 16;  // arrival counters per handle (256 KiB)
 
 void log_error(const char* op, const char* fmt, ...);
 

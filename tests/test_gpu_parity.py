@@ -538,6 +538,34 @@ def test_small_kernel_single_wave(gpu, ta, tb):
         assert plan["path"] == 5 and plan["parts"] == slices, plan
 
 
+def test_small_kernel_sliced_calls_back_to_back(gpu):
+    """k-sliced one-launch calls share the handle's arrival counters, which each launch
+    leaves at zero: calls of different tile counts queued back to back (no synchronisation
+    between them) all meet the oracle, and repeating the first gives its bytes again."""
+    import torch
+
+    import cgemm3m
+
+    shapes = [(70, 50, 900), (300, 200, 640), (70, 50, 900), (33, 400, 1200)]
+    runs = []
+    for idx, (m, n, k) in enumerate(shapes):
+        A, B, C = inputs("N", "C", m, n, k, m, n, m, 20262000 + m + n + k)
+        dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+        cfg = cgemm3m.config(variant=5, k_split=3 + idx % 2)
+        st = cgemm3m.cgemm3m(gpu, "N", "C", m, n, k, ALPHA2, ptr(dA), m, ptr(dB), n, BETA2, ptr(dC), m, cfg)
+        assert st == 0
+        runs.append((m, n, k, A, B, C, dA, dB, dC))
+    torch.cuda.synchronize()
+    outs = []
+    for m, n, k, A, B, C, dA, dB, dC in runs:
+        out = dC.cpu().numpy()
+        gold = oracle.cgemm("N", "C", m, n, k, ALPHA2, A, m, B, n, BETA2, C, m)
+        check_against_oracle(out, gold, C, m, n, m, "sliced-back-to-back")
+        outs.append(out)
+    same = outs[0].tobytes() == outs[2].tobytes()
+    assert same
+
+
 def test_small_kernel_auto_slices(gpu):
     """Few 16x16 tiles and a long k: the planner slices k on its own (about two workgroups
     per CU)."""
