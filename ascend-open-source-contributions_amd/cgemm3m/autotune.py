@@ -182,7 +182,7 @@ def read_trace(trace, manifest):
             if cur is None:
                 cur = []
                 groups.append(cur)
-            cur.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+            cur.append((r["Kernel_Name"], (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6))
         else:
             cur = None
     want = sum(len(e["cands"]) for e in man["entries"])
@@ -194,6 +194,9 @@ def read_trace(trace, manifest):
         for c in e["cands"]:
             ks = groups[g]
             g += 1
+            if len(ks) % calls == 1 and "fill_u64_kernel" in ks[0][0]:
+                ks = ks[1:]  # the handle's one-time clear of its arrival-counter pool
+            ks = [t for _, t in ks]
             if len(ks) % calls:
                 raise SystemExit("shape %s candidate %s: %d kernels for %d calls" % (e["shape"], c, len(ks), calls))
             per = len(ks) // calls
