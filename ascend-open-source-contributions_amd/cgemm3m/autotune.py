@@ -224,6 +224,35 @@ def pick_bins(measured):
     return out
 
 
+def pick_shapes(measured, single):
+    """Per measured shape, its own pick (same rule as `choose`) where it differs from its
+    bin's: [((ta, tb, m, n, k), (tile, parts))], first measurement of a repeated shape kept."""
+    bins = {b: best for (b, _, best) in single}
+    out, seen = [], set()
+    for shape, res in measured:
+        if shape in seen:
+            continue
+        seen.add(shape)
+        best = choose(res)
+        if bins.get(bin_of(*shape[2:5])) != best:
+            out.append((shape, best))
+    return out
+
+
+def read_grouped_entries(path=TABLE):
+    """The grouped class entries of an existing table: [((mnLog2, kLog2), 1, (tile, parts))]."""
+    import re
+
+    text = open(path).read()
+    body = text[text.index("kTunedTable[]"):]
+    body = body[:body.index("};")]
+    out = []
+    for a, b, g, t, p in re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", body):
+        if int(g) == 1 and int(t) != 0:
+            out.append(((int(a), int(b)), 1, (int(t), int(p))))
+    return out
+
+
 def tune_grouped(h, torch, shapes, reps, count=64):
     """Grouped classes: one grouped call of `count` identical problems per candidate, with
     the member plan pinned through aclblasCgemm3mGroupedBatchedEx."""
@@ -260,16 +289,18 @@ def tune_grouped(h, torch, shapes, reps, count=64):
     return out
 
 
-def write_table(entries, path=TABLE):
+def write_table(entries, path=TABLE, shapes=()):
     seen = {}
     for (mnl, kl), grouped, (tile, parts) in entries:
         seen[(mnl, kl, grouped)] = (tile, parts)
+    ops = {"N": 111, "T": 112, "C": 113}
     lines = [
         "// tuned_table.h -- plan table written by cgemm3m/autotune.py from MI355X measurements.",
-        "// Key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and K partitions",
-        "// (tile 32 / 16: the one-launch small-problem kernel, four waves / one wave per 16x16 tile;",
-        "// parts = k slices).",
-        "// Bins not listed fall back to the planner's occupancy rule.",
+        "// kTunedTable key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and",
+        "// K partitions (tile 32 / 16: the one-launch small-problem kernel, four waves / one wave per",
+        "// 16x16 tile; parts = k slices). Bins not listed fall back to the planner's occupancy rule.",
+        "// kTunedShapes: one-problem shapes the tuner measured whose own best plan differs from their",
+        "// bin's, looked up first by exact (opA, opB, m, n, k).",
         "#pragma once",
         "",
         "namespace c3m {",
@@ -278,11 +309,19 @@ def write_table(entries, path=TABLE):
         "    short mnLog2, kLog2, grouped, tile, parts;",
         "};",
         "",
+        "struct TunedShape {",
+        "    int m, n, k;",
+        "    short opA, opB, tile, parts;",
+        "};",
+        "",
         "static const TunedEntry kTunedTable[] = {",
     ]
     for (mnl, kl, g), (tile, parts) in sorted(seen.items()):
         lines.append("    {%d, %d, %d, %d, %d}," % (mnl, kl, g, tile, parts))
-    lines += ["    {0, 0, 0, 0, 0},  // sentinel", "};", "", "}  // namespace c3m", ""]
+    lines += ["    {0, 0, 0, 0, 0},  // sentinel", "};", "", "static const TunedShape kTunedShapes[] = {"]
+    for (ta, tb, m, n, k), (tile, parts) in sorted(shapes, key=lambda e: (e[0][2], e[0][3], e[0][4], e[0][0], e[0][1])):
+        lines.append("    {%d, %d, %d, %d, %d, %d, %d}," % (m, n, k, ops[ta], ops[tb], tile, parts))
+    lines += ["    {0, 0, 0, 0, 0, 0, 0},  // sentinel", "};", "", "}  // namespace c3m", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))
     return path
@@ -297,6 +336,9 @@ def main():
     ap.add_argument("--write", action="store_true")
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--keep-grouped", action="store_true",
+                    help="pick: reuse the grouped entries of the installed table instead of timing them "
+                         "(no GPU needed)")
     args = ap.parse_args()
     sys.path.insert(0, os.path.dirname(HERE))
     if args.mode == "pick" and len(args.paths) != 2:
@@ -315,16 +357,21 @@ def main():
         run_trace(h, torch, shapes, args.paths[0])
         h.close()
         return
-    if args.mode == "pick":
-        single = pick_bins(read_trace(args.paths[0], args.paths[1]))
-    else:
-        single = pick_bins(tune_single_events(h, torch, shapes, args.reps))
+    measured = read_trace(args.paths[0], args.paths[1]) if args.mode == "pick" else \
+        tune_single_events(h, torch, shapes, args.reps)
+    single = pick_bins(measured)
     for (b, _, best) in single:
         print("bin mn 2^%d k 2^%d: tile %d parts %d" % (b[0], b[1], best[0], best[1]))
-    grouped = GROUPED_SHAPES[::3] if args.quick else GROUPED_SHAPES
-    entries = single + tune_grouped(h, torch, grouped, args.reps)
+    own = pick_shapes(measured, single)
+    for shape, best in own:
+        print("shape %s %s %d x %d x %d: tile %d parts %d" % (shape + best))
+    if args.keep_grouped:
+        grouped_entries = read_grouped_entries()
+    else:
+        grouped = GROUPED_SHAPES[::3] if args.quick else GROUPED_SHAPES
+        grouped_entries = tune_grouped(h, torch, grouped, args.reps)
     if args.write:
-        print("wrote", write_table(entries, args.out))
+        print("wrote", write_table(single + grouped_entries, args.out, own))
     h.close()
 
 
