@@ -300,7 +300,8 @@ def write_table(entries, path=TABLE, shapes=()):
         "// K partitions (tile 32 / 16: the one-launch small-problem kernel, four waves / one wave per",
         "// 16x16 tile; parts = k slices). Bins not listed fall back to the planner's occupancy rule.",
         "// kTunedShapes: one-problem shapes the tuner measured whose own best plan differs from their",
-        "// bin's, looked up first by exact (opA, opB, m, n, k).",
+        "// bin's, looked up first by exact (opA, opB, m, n, k); a packed-tile entry (tile 64 / 128)",
+        "// also serves the same (m, n, k) under other ops.",
         "#pragma once",
         "",
         "namespace c3m {",

@@ -24,14 +24,26 @@ namespace c3m {
 bool tuned_lookup(int opA, int opB, long long m, long long n, long long k, bool grouped, int* tile, int* parts)
 {
     if (m <= 0 || n <= 0 || k <= 0) return false;
-    if (!grouped) {  // a shape the tuner measured itself, where its plan differs from its class's
+    if (!grouped) {
+        // A shape the tuner measured itself, where its plan differs from its class's: the same
+        // ops first; else, for a packed-tile plan, the same sizes under other ops (the pack
+        // turns every op into the same planes, so the packed plans rank alike whatever the ops;
+        // the one-launch kernel reads the operands as stored, so its entries need their ops).
+        const TunedShape* any = nullptr;
         for (const TunedShape& e : kTunedShapes) {
             if (e.tile == 0) break;
-            if (e.m == m && e.n == n && e.k == k && e.opA == opA && e.opB == opB) {
+            if (e.m != m || e.n != n || e.k != k) continue;
+            if (e.opA == opA && e.opB == opB) {
                 *tile = e.tile;
                 *parts = e.parts;
                 return true;
             }
+            if (any == nullptr && e.tile >= 64) any = &e;
+        }
+        if (any != nullptr) {
+            *tile = any->tile;
+            *parts = any->parts;
+            return true;
         }
     }
     int mnl = 0, kl = 0;
