@@ -285,3 +285,42 @@ def test_planner_follows_tuned_shapes_and_classes(h):
         checked += 1
     assert checked >= 20, checked
 
+
+def test_planner_random_arguments(h):
+    """Random sizes, ops and configs (valid and not) through the host planner and the grouped
+    layout: only documented statuses come back, and every plain call on valid arguments gets
+    a plan (NOT_INITIALIZED without a device means the plan was made)."""
+    import random
+
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("needs a host without a GPU (the calls must stop after planning)")
+    rng = random.Random(11)
+
+    def dim():
+        r = rng.random()
+        return rng.choice([0, 1, 2, 15, 16, 17, 33]) if r < 0.1 else rng.randint(1, 3000 if r < 0.85 else 70000)
+
+    for _ in range(1500):
+        m, n, k = dim(), dim(), dim()
+        if m * n > 1 << 28:
+            n = max(1, (1 << 28) // m)
+        ta, tb = rng.choice("NTC"), rng.choice("NTC")
+        plain = rng.random() < 0.3
+        cfg = None if plain else cgemm3m.config(
+            rng.choice(["fp32", "bf16x3", "bf16x6", "fp16"]), tile=rng.choice([0, 0, 16, 32, 64, 128, 7]),
+            k_split=rng.choice([0, 0, 1, 2, 3, 4, 16, 17]), correction=rng.choice([0, -1, 1, 2, 3]),
+            identity=rng.choice([0, 1, 2, 3]), variant=rng.choice([0, 0, 1, 2, 3, 4, 5, 6, 7]))
+        lda = max(1, m if ta == "N" else k)
+        ldb = max(1, k if tb == "N" else n)
+        st = cgemm3m.cgemm3m(h, ta, tb, m, n, k, complex(0.5, 1), FAKE, lda, FAKE, ldb, 0.5, FAKE, max(1, m), cfg,
+                             check=False)
+        assert st in (_lib.SUCCESS, 1, _lib.INVALID_VALUE), (st, m, n, k, ta, tb, cfg and list(cfg._fields_))
+        if plain:
+            assert st == (_lib.SUCCESS if m == 0 or n == 0 else 1), (st, m, n, k)
+    for _ in range(150):
+        probs = [dict(ta=rng.choice("NTC"), tb=rng.choice("NTC"), m=rng.randint(0, 600), n=rng.randint(0, 600),
+                      k=rng.randint(0, 2500)) for _ in range(rng.randint(1, 40))]
+        assert cgemm3m.grouped_workspace_size(h, probs) >= 0
+
