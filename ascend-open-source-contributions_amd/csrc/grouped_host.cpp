@@ -60,14 +60,20 @@ bool tuned_lookup(int opA, int opB, long long m, long long n, long long k, bool 
     return false;
 }
 
-static uint64_t fnv1a(const void* data, size_t bytes)
+// Content key of a descriptor block, 8 bytes at a time: each word is folded in with a
+// multiply and a shift (the key only selects a cached block; memcmp confirms the match).
+static uint64_t block_hash(const void* data, size_t bytes)
 {
     const unsigned char* p = static_cast<const unsigned char*>(data);
-    uint64_t h = 1469598103934665603ull;
-    for (size_t i = 0; i < bytes; ++i) {
-        h ^= p[i];
-        h *= 1099511628211ull;
+    uint64_t h = 0x9e3779b97f4a7c15ull ^ bytes;
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h = (h ^ w) * 0xff51afd7ed558ccdull;
+        h ^= h >> 32;
     }
+    for (; i < bytes; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
     return h;
 }
 
@@ -82,7 +88,7 @@ static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>&
                                      hipError_t* err)
 {
     const bool capturing = stream_capturing(s);
-    const uint64_t key = fnv1a(blob.data(), blob.size());
+    const uint64_t key = block_hash(blob.data(), blob.size());
     size_t idx = h->staged.size();
     for (size_t i = 0; i < h->staged.size(); ++i) {
         const auto& st = h->staged[i];
