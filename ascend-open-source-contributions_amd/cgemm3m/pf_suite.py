@@ -1,5 +1,5 @@
 """The reference's performance suite on one MI355X: the 200 TC_PF cases (alpha = 1, beta = 0,
-shapes/ops/leading dimensions and the task-provided GPU baseline times from
+shapes/ops/leading dimensions and the reference's GPU baseline times from
 tests/golden/pf_cases.csv, which holds the reference's ops-blas/test/gemm/cgemm3m/arch35/
 gpu_baseline.csv).
 

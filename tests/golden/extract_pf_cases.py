@@ -1,5 +1,5 @@
 """Writes pf_cases.csv here: the reference's 200 performance cases (TC_PF, alpha = 1, beta = 0)
-with the task-provided GPU baseline time of each, taken verbatim from the reference's data file
+with the reference GPU baseline time of each, taken verbatim from the reference's data file
 ops-blas/test/gemm/cgemm3m/arch35/gpu_baseline.csv (columns id, transa, transb, m, n, k, lda,
 ldb, ldc, gpu_ms). The reference times each case as the sum of its kernels' device durations
 per API call, mean of 10 calls after 5 warm-up calls (ops-blas/test/gemm/cgemm3m/README.md:
