@@ -61,6 +61,31 @@ def test_8192_forms_errors_and_reruns(gpu, ops):
     assert errs["fp16"]["normwise"] < 2e-3, errs["fp16"]
 
 
+def test_8192_row_scaled_bf16x3_within_twice_fp32(gpu):
+    """The range-stress variant: rows of op(A) scaled by 10^u, u ~ U[-3, 3] (N,N). bf16x3's
+    max elementwise and normwise FP64 errors stay within twice the plain path's."""
+    import torch
+
+    import cgemm3m
+
+    A, B, C0 = _inputs(torch, 20261850)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(20261851)
+    scale = torch.pow(10.0, (torch.rand(N, generator=g, device="cuda") * 6 - 3)).to(torch.complex64)
+    A = (A.view(N, N) * scale[None, :]).reshape(-1)  # column-major: row i of A is A.view(N, N)[:, i]
+    errs = {}
+    for form in ("fp32", "bf16x3"):
+        C = C0.clone()
+        st = cgemm3m.cgemm3m(gpu, "N", "N", N, N, N, ALPHA, ptr(A), N, ptr(B), N, BETA, ptr(C), N,
+                             cgemm3m.config(form))
+        assert st == 0
+        errs[form] = cgemm3m.residual(gpu, "N", "N", N, N, N, ALPHA, ptr(A), N, ptr(B), N, BETA, ptr(C0), ptr(C), N)
+        print(form, "row-scaled", errs[form])
+    fp, bf = errs["fp32"], errs["bf16x3"]
+    assert bf["max_rel"] <= 2.0 * fp["max_rel"], (bf, fp)
+    assert bf["normwise"] <= 2.0 * fp["normwise"], (bf, fp)
+
+
 @pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
 def test_8192_integer_inputs_bitwise_sampled(gpu, form):
     """Integer real and imaginary parts in [-8, 8]: |partial sums| <= 16 * 16 * 8192 < 2^24,
