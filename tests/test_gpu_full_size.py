@@ -115,3 +115,58 @@ def test_8192_integer_inputs_bitwise_sampled(gpu, form):
     want = gold.view(np.float32) + np.float32(0.0)
     same = got.tobytes() == want.tobytes()
     assert same, "%s: %d sampled outputs differ" % (form, int(np.count_nonzero(out != gold)))
+
+
+def test_grouped_512_batch(gpu):
+    """The grouped configuration at full size: 512 problems, m, n uniform over multiples of 16
+    in [32, 512], k over multiples of 64 in [64, 2048], ops uniform over {N,T,C}^2, alpha = 1,
+    beta = 0. Every member's FP64 normwise error is within the FP32 bound, a rerun is
+    byte-identical, and replays of a captured graph equal the eager call byte for byte."""
+    import torch
+
+    import cgemm3m
+
+    rng = np.random.default_rng(4243)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4243)
+    probs, keep = [], []
+    for _ in range(512):
+        ta, tb = str(rng.choice(list("NTC"))), str(rng.choice(list("NTC")))
+        m, n, k = int(rng.integers(2, 33)) * 16, int(rng.integers(2, 33)) * 16, int(rng.integers(1, 33)) * 64
+        A = (torch.rand(2 * m * k, generator=g, device="cuda") * 2 - 1).view(torch.complex64)
+        B = (torch.rand(2 * k * n, generator=g, device="cuda") * 2 - 1).view(torch.complex64)
+        C = torch.zeros(m * n, dtype=torch.complex64, device="cuda")
+        keep.append((A, B, C))
+        probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=1.0, A=ptr(A), lda=m if ta == "N" else k, B=ptr(B),
+                          ldb=k if tb == "N" else n, beta=0.0, C=ptr(C), ldc=m))
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    first = [c.clone() for (_, _, c) in keep]
+    zero = torch.zeros(512 * 512, dtype=torch.complex64, device="cuda")
+    worst = 0.0
+    for p, (A, B, C) in zip(probs, keep):
+        r = cgemm3m.residual(gpu, p["ta"], p["tb"], p["m"], p["n"], p["k"], 1.0, p["A"], p["lda"], p["B"], p["ldb"],
+                             0.0, ptr(zero), ptr(C), p["ldc"])
+        worst = max(worst, r["normwise"])
+        assert r["normwise"] < 4e-6, (p["ta"], p["tb"], p["m"], p["n"], p["k"], r)
+    print("grouped 512: worst member normwise error %.3g" % worst)
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    again = sum(not torch.equal(c.view(torch.int64), f.view(torch.int64)) for (_, _, c), f in zip(keep, first))
+    assert again == 0, "%d members differ on a rerun" % again
+    stream = torch.cuda.current_stream()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gpu.set_stream(torch.cuda.current_stream())
+        cgemm3m.grouped(gpu, probs)
+    gpu.set_stream(stream)
+    for rep in range(2):
+        for (_, _, c) in keep:
+            c.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        bad = sum(not torch.equal(c.view(torch.int64), f.view(torch.int64)) for (_, _, c), f in zip(keep, first))
+        assert bad == 0, "replay %d: %d members differ" % (rep, bad)
+    del graph
+
