@@ -207,11 +207,15 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
                                    tuned_lookup((int)p.ta, (int)p.tb, m, n, k, false, &tt, &tp) && tt >= 64) {
         pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
-        // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition.
+        // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition. FP32:
+        // one workgroup per CU, at most 16 partitions. 16-bit formats (the fused kernel, whose
+        // long k loops are latency-bound at one workgroup per CU): about two per CU, at most 64
+        // (256 x 256 x 65536 bf16x3: 1.57 ms at 16 partitions).
         pl.parts = 1;
+        const long long perCu = pl.format == FMT_F32 ? 1 : 2, cap = pl.format == FMT_F32 ? 16 : 64;
         if (tiles < cus) {
-            long long parts = (cus + tiles - 1) / tiles;
-            parts = std::min<long long>(parts, 16);
+            long long parts = (perCu * cus + tiles - 1) / tiles;
+            parts = std::min<long long>(parts, cap);
             parts = std::min<long long>(parts, std::max(1, pl.kSteps * pl.bk / 256));
             pl.parts = (int)std::max<long long>(parts, 1);
         }
