@@ -193,34 +193,45 @@ hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blo
 // FP16 block scale: exponent e per (plane, 128 mn, 128 k) block with max|x| * 2^-e in
 // [2^14, 2^15), so the block's largest value keeps full FP16 precision without overflow;
 // kScaleNone for a block that holds only zeros.
-__global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)
+// One 1024-thread workgroup per (128 mn, 128 k) block: 16 elements per thread, loaded four at
+// a time (clamped in-range addresses, out-of-range elements count as zero, no branch around a
+// load), then the maxima reduced within each wave and over the 16 waves (max is order-free).
+__global__ __launch_bounds__(1024) void scale_exp_kernel(ScaleArgs a)
 {
-    __shared__ float red[3][256];
+    __shared__ float red[3][16];
     const int mb = blockIdx.x, kb = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const float2* src = reinterpret_cast<const float2*>(a.src);
     float mx[3] = {0.0f, 0.0f, 0.0f};
-    for (int e = threadIdx.x; e < 128 * 128; e += 256) {
-        const int fast = e & 127, slow = e >> 7;
-        const int r = mb * 128 + (a.mnContig ? fast : slow);
-        const int c = kb * 128 + (a.mnContig ? slow : fast);
-        if (r < a.mn && c < a.kk) {
-            float2 v = a.mnContig ? src[(size_t)c * a.ld + r] : src[(size_t)r * a.ld + c];
-            if (a.conj) v.y = -v.y;
+    for (int grp = 0; grp < 4; ++grp) {
+        float2 v[4];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) mx[q] = fmaxf(mx[q], fabsf(plane_value(a.role, a.identity, q, v.x, v.y)));
+        for (int t = 0; t < 4; ++t) {
+            const int e = threadIdx.x + 1024 * (grp * 4 + t);
+            const int fast = e & 127, slow = e >> 7;
+            const int r = mb * 128 + (a.mnContig ? fast : slow);
+            const int c = kb * 128 + (a.mnContig ? slow : fast);
+            const int rr = min(r, a.mn - 1), cc = min(c, a.kk - 1);
+            v[t] = a.mnContig ? src[(size_t)cc * a.ld + rr] : src[(size_t)rr * a.ld + cc];
+            const bool in = r < a.mn && c < a.kk;
+            v[t].x = in ? v[t].x : 0.0f;
+            v[t].y = in ? (a.conj ? -v[t].y : v[t].y) : 0.0f;
         }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) mx[q] = fmaxf(mx[q], fabsf(plane_value(a.role, a.identity, q, v[t].x, v[t].y)));
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = mx[q];
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx[q] = fmaxf(mx[q], __shfl_xor(mx[q], off));
+        if (lane == 0) red[q][wave] = mx[q];
+    }
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) red[q][threadIdx.x] = fmaxf(red[q][threadIdx.x], red[q][threadIdx.x + w]);
-        __syncthreads();
-    }
     if (threadIdx.x < 3) {
-        const float m = red[threadIdx.x][0];
+        float m = red[threadIdx.x][0];
+        for (int w = 1; w < 16; ++w) m = fmaxf(m, red[threadIdx.x][w]);
         int e = m == 0.0f ? kScaleNone : 0;
         if (m > 0.0f && isfinite(m)) {
             int ex;
@@ -234,7 +245,7 @@ __global__ __launch_bounds__(256) void scale_exp_kernel(ScaleArgs a)
 
 hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(scale_exp_kernel, dim3(a.mnPad / 128, (a.kPad + 127) / 128), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(scale_exp_kernel, dim3(a.mnPad / 128, (a.kPad + 127) / 128), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 
