@@ -452,11 +452,11 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         const PackArgs pb = pack_args(p, pl, ws, 1);
         if (pl.format == FMT_F16) {
             StageTimer t(h, ST_PACK);
-            for (const PackArgs* x : {&pa, &pb}) {
-                ScaleArgs sa{x->src, x->ld, x->mn, x->kk, x->mnPad, x->kPad, x->mnContig, x->conj, x->role,
-                             x->identity, const_cast<int*>(x->scaleExp)};
-                if (e == hipSuccess) e = launch_scale_exponents(sa, s);
-            }
+            auto scale_args = [](const PackArgs& x) {
+                return ScaleArgs{x.src, x.ld, x.mn, x.kk, x.mnPad, x.kPad, x.mnContig, x.conj, x.role,
+                                 x.identity, const_cast<int*>(x.scaleExp)};
+            };
+            if (e == hipSuccess) e = launch_scale_exponents(scale_args(pa), scale_args(pb), s);
             ScaleSummaryArgs sm{const_cast<int*>(pa.scaleExp), 3 * (pl.mPad / 128), const_cast<int*>(pb.scaleExp),
                                 3 * (pl.nPad / 128),
                                 (pl.kPad + 127) / 128, reinterpret_cast<int*>(ws + pl.offSumA),

@@ -8,7 +8,7 @@ namespace c3m {
 
 hipError_t launch_pack(const PackArgs& a, hipStream_t s);
 hipError_t launch_pack_pair(const PackArgs& a, const PackArgs& b, hipStream_t s);
-hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s);
+hipError_t launch_scale_exponents(const ScaleArgs& a, const ScaleArgs& b, hipStream_t s);  // A and B, one launch
 hipError_t launch_scale_summary(const ScaleSummaryArgs& a, hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_gemm_f32(const GemmArgs& g, int bm, int bn, int variant, hipStream_t s);

@@ -196,10 +196,12 @@ hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blo
 // One 1024-thread workgroup per (128 mn, 128 k) block: 16 elements per thread, loaded four at
 // a time (clamped in-range addresses, out-of-range elements count as zero, no branch around a
 // load), then the maxima reduced within each wave and over the 16 waves (max is order-free).
-__global__ __launch_bounds__(1024) void scale_exp_kernel(ScaleArgs a)
+__global__ __launch_bounds__(1024) void scale_exp_kernel(ScaleArgs a0, ScaleArgs a1)
 {
     __shared__ float red[3][16];
+    const ScaleArgs& a = blockIdx.z == 0 ? a0 : a1;  // grid z: operand A, then B
     const int mb = blockIdx.x, kb = blockIdx.y;
+    if (mb >= (a.mnPad >> 7)) return;  // the grid covers the larger operand's row blocks
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const float2* src = reinterpret_cast<const float2*>(a.src);
     float mx[3] = {0.0f, 0.0f, 0.0f};
@@ -243,9 +245,11 @@ __global__ __launch_bounds__(1024) void scale_exp_kernel(ScaleArgs a)
     }
 }
 
-hipError_t launch_scale_exponents(const ScaleArgs& a, hipStream_t s)
+hipError_t launch_scale_exponents(const ScaleArgs& a, const ScaleArgs& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(scale_exp_kernel, dim3(a.mnPad / 128, (a.kPad + 127) / 128), dim3(1024), 0, s, a);
+    if (a.kPad != b.kPad) return hipErrorInvalidValue;  // one grid over both operands' k blocks
+    const int rows = (a.mnPad > b.mnPad ? a.mnPad : b.mnPad) / 128;
+    hipLaunchKernelGGL(scale_exp_kernel, dim3(rows, (a.kPad + 127) / 128, 2), dim3(1024), 0, s, a, b);
     return hipGetLastError();
 }
 
