@@ -104,6 +104,7 @@ def summary(trace_csv, out_md, path=CASES):
                                    for name, _ in g[:per]}))
         lines.append((c, ours, ratio, per, kernels))
     gmean = math.exp(sum(math.log(x) for x in ratios) / len(ratios))
+    trace_dir = os.path.dirname(os.path.abspath(trace_csv))
     out = ["# The reference's 200 performance cases on one MI355X", "",
            "Cases, GPU baseline times and method from the reference (`ops-blas/test/gemm/cgemm3m/arch35/"
            "gpu_baseline.csv`, `ops-blas/test/gemm/cgemm3m/README.md:95-115`): alpha = 1, beta = 0; per case 5 "
@@ -112,10 +113,10 @@ def summary(trace_csv, out_md, path=CASES):
            "samples; ratio = GPU baseline time / MI355X time (the reference's own device gate is ratio >= 0.4 per "
            "case, mean >= 0.5).", "",
            "Commands (MI355X, one GPU):", "", "```",
-           "cd /tmp && TMPDIR=/tmp rocprofv3 --kernel-trace --output-format csv -d /root/repo/gpurun_out/pf -o run -- "
-           "python3 -m cgemm3m.pf_suite run   # from ascend-open-source-contributions_amd/",
-           "python3 -m cgemm3m.pf_suite summary /root/repo/gpurun_out/pf/run_kernel_trace.csv "
-           "/root/repo/profiles/pf_suite_vs_gpu_baseline.md", "```", "",
+           "cd /tmp && TMPDIR=/tmp rocprofv3 --kernel-trace --output-format csv -d %s -o run -- "
+           "python3 -m cgemm3m.pf_suite run   # from ascend-open-source-contributions_amd/" % trace_dir,
+           "python3 -m cgemm3m.pf_suite summary %s %s" % (os.path.abspath(trace_csv), os.path.abspath(out_md)),
+           "```", "",
            "| | value |", "|---|---|",
            "| cases | %d |" % len(ratios),
            "| ratio min | %.3f |" % min(ratios),
