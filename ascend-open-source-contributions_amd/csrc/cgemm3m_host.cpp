@@ -62,41 +62,41 @@ aclblasStatus_t validate(aclblasHandle_t h, aclblasOperation_t ta, aclblasOperat
                          const char* op)
 {
     if (h == nullptr) {
-        log_error(op, "handle is nullptr");
+        log_error(op, "no handle was given (create one with aclblasCreate)");
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
     if (!is_op(ta)) {
-        log_error(op, "invalid transa: %d", (int)ta);
+        log_error(op, "op(A) code %d is not N (111), T (112) or C (113)", (int)ta);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (!is_op(tb)) {
-        log_error(op, "invalid transb: %d", (int)tb);
+        log_error(op, "op(B) code %d is not N (111), T (112) or C (113)", (int)tb);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (m < 0 || n < 0 || k < 0) {
-        log_error(op, "m, n, k must be non-negative: m=%d, n=%d, k=%d", m, n, k);
+        log_error(op, "negative dimension in (m, n, k) = (%d, %d, %d)", m, n, k);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     const int ldaMin = ta != ACLBLAS_OP_N ? std::max(1, k) : std::max(1, m);
     if (lda < ldaMin) {
-        log_error(op, "lda=%d must be >= %d", lda, ldaMin);
+        log_error(op, "lda=%d is below the stored rows of A (%d)", lda, ldaMin);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     const int ldbMin = tb != ACLBLAS_OP_N ? std::max(1, n) : std::max(1, k);
     if (ldb < ldbMin) {
-        log_error(op, "ldb=%d must be >= %d", ldb, ldbMin);
+        log_error(op, "ldb=%d is below the stored rows of B (%d)", ldb, ldbMin);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (ldc < std::max(1, m)) {
-        log_error(op, "ldc=%d must be >= %d", ldc, std::max(1, m));
+        log_error(op, "ldc=%d is below the rows of C (%d)", ldc, std::max(1, m));
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (alpha == nullptr) {
-        log_error(op, "alpha must not be nullptr");
+        log_error(op, "no alpha scalar was given");
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (beta == nullptr) {
-        log_error(op, "beta must not be nullptr");
+        log_error(op, "no beta scalar was given");
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     return ACLBLAS_STATUS_SUCCESS;
@@ -106,11 +106,11 @@ aclblasStatus_t validate(aclblasHandle_t h, aclblasOperation_t ta, aclblasOperat
 aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op)
 {
     if (C == nullptr) {
-        log_error(op, "C must not be nullptr for a non-empty output");
+        log_error(op, "C is a null pointer but the output is not empty");
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     if (k > 0 && (A == nullptr || B == nullptr)) {
-        log_error(op, "A and B must not be nullptr when k > 0");
+        log_error(op, "k = %d needs both A and B, and one of them is a null pointer", k);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     return ACLBLAS_STATUS_SUCCESS;
@@ -683,7 +683,7 @@ extern "C" aclblasStatus_t aclblasCgemm(aclblasHandle_t handle, aclblasOperation
 {
     const char* op = "aclblasCgemm";
     if (handle == nullptr) {
-        log_error(op, "handle is nullptr");
+        log_error(op, "no handle was given (create one with aclblasCreate)");
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
     const auto valid = [](aclblasOperation_t t) {

@@ -43,7 +43,7 @@ extern "C" aclblasStatus_t aclblasCgemm3mStridedBatched(aclblasHandle_t handle, 
 {
     const char* op = "aclblasCgemm3mStridedBatched";
     if (handle == nullptr) {
-        log_error(op, "handle is nullptr");
+        log_error(op, "no handle was given (create one with aclblasCreate)");
         return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     }
     // Unlike aclblasCgemm3m, this entry reports a bad op as INVALID_ENUM.
@@ -52,7 +52,7 @@ extern "C" aclblasStatus_t aclblasCgemm3mStridedBatched(aclblasHandle_t handle, 
         return ACLBLAS_STATUS_INVALID_ENUM;
     }
     if (batchCount < 0) {
-        log_error(op, "batchCount must be non-negative: %d", batchCount);
+        log_error(op, "batchCount %d is negative", batchCount);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     aclblasStatus_t st = validate(handle, transa, transb, m, n, k, alpha, lda, ldb, beta, ldc, op);
