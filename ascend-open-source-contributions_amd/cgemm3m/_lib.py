@@ -26,7 +26,7 @@ EXPORTS = [
     "aclblasCgemm3mGetWorkspaceSize",
     "aclblasCgemm3mGroupedBatched", "aclblasCgemm3mGroupedBatchedEx", "aclblasCgemm3mGroupedBatchedGetWorkspaceSize",
     "aclblasCgemm3mResidual", "aclblasCgemm3mGetLastPlan", "aclblasCgemm3mSetTiming",
-    "aclblasCgemm3mGetTiming", "aclblasSetLogCallback",
+    "aclblasCgemm3mGetTiming", "aclblasSetLogCallback", "aclblasCgemm3mNoAlloc",
 ]
 STAGES = ["pack", "energy", "gemm", "merge", "recompute", "scale"]
 
@@ -63,6 +63,7 @@ def load():
     L.aclblasCgemm3m.argtypes = gemm
     L.aclblasCgemm.argtypes = gemm
     L.aclblasCgemm3mEx.argtypes = gemm + [ctypes.POINTER(Config)]
+    L.aclblasCgemm3mNoAlloc.argtypes = gemm
     i64 = ctypes.c_int64
     L.aclblasCgemm3mStridedBatched.argtypes = [vp, i, i, i, i, i, cp, vp, i, i64, vp, i, i64, cp, vp, i, i64, i]
     L.aclblasCgemm3mGetWorkspaceSize.argtypes = [vp, i, i, i, i, i, cp, cp, ctypes.POINTER(Config),

@@ -32,7 +32,7 @@ static int default_variant(int format, int bm)
 
 StageTimer::StageTimer(aclblasHandle_t h, int stage) : h_(h), stage_(stage)
 {
-    if (h_->timing && hipEventCreate(&start_) == hipSuccess) {
+    if (h_ != nullptr && h_->timing && hipEventCreate(&start_) == hipSuccess) {
         (void)hipEventRecord(start_, h_->stream);
     } else {
         start_ = nullptr;
@@ -393,9 +393,11 @@ static PackArgs pack_args(const Problem& p, const Plan& pl, char* ws, int role)
     return a;
 }
 
-aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, char* ws, const char* op)
+aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, char* ws, const char* op, bool noAlloc)
 {
     hipStream_t s = h->stream;
+    // With noAlloc the call creates nothing: no counter pool, no timing events.
+    aclblasHandle_t th = noAlloc ? nullptr : h;
     hipError_t e = hipSuccess;
     h->lastCounter = nullptr;
     if (pl.path == 0) return ACLBLAS_STATUS_SUCCESS;
@@ -411,7 +413,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         unsigned int* arrive = nullptr;
         bool clear = false;
         if (sliced) {
-            if (pl.smallTiles <= kArrivePool && h->arrivePool == nullptr && !stream_capturing(s)) {
+            if (!noAlloc && pl.smallTiles <= kArrivePool && h->arrivePool == nullptr && !stream_capturing(s)) {
                 void* pool = nullptr;
                 if (hipMalloc(&pool, (size_t)kArrivePool * sizeof(unsigned int)) == hipSuccess) {
                     e = launch_fill_u64(static_cast<unsigned long long*>(pool), kArrivePool / 2, s);
@@ -423,7 +425,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
                     (void)hipGetLastError();  // fall back to the workspace's counters
                 }
             }
-            if (pl.smallTiles <= kArrivePool && h->arrivePool != nullptr) {
+            if (!noAlloc && pl.smallTiles <= kArrivePool && h->arrivePool != nullptr) {
                 arrive = h->arrivePool;
             } else {
                 arrive = reinterpret_cast<unsigned int*>(ws + pl.offArrive);
@@ -435,7 +437,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
                     pl.correction, amag * 0x1p-24, 0x1p-8, slots, pl.parts, pl.smallWaves,
                     sliced ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr,
                     sliced && pl.correction ? reinterpret_cast<double*>(ws + pl.offEPA) : nullptr, arrive};
-        StageTimer t(h, ST_GEMM);
+        StageTimer t(th, ST_GEMM);
         if (e == hipSuccess && clear)
             e = launch_fill_u64(reinterpret_cast<unsigned long long*>(arrive), (pl.smallTiles + 1) / 2, s);
         if (e == hipSuccess) e = launch_gemm_small(a, s);
@@ -445,13 +447,13 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         }
     } else if (pl.path == 3) {
         ScaleCArgs a{p.m, p.n, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0, p.C, p.ldc};
-        StageTimer t(h, ST_SCALE);
+        StageTimer t(th, ST_SCALE);
         e = launch_scale_c(a, s);
     } else {
         const PackArgs pa = pack_args(p, pl, ws, 0);
         const PackArgs pb = pack_args(p, pl, ws, 1);
         if (pl.format == FMT_F16) {
-            StageTimer t(h, ST_PACK);
+            StageTimer t(th, ST_PACK);
             auto scale_args = [](const PackArgs& x) {
                 return ScaleArgs{x.src, x.ld, x.mn, x.kk, x.mnPad, x.kPad, x.mnContig, x.conj, x.role,
                                  x.identity, const_cast<int*>(x.scaleExp)};
@@ -464,7 +466,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             if (e == hipSuccess) e = launch_scale_summary(sm, s);
         }
         if (e == hipSuccess) {
-            StageTimer t(h, ST_PACK);
+            StageTimer t(th, ST_PACK);
             e = launch_pack_pair(pa, pb, s);
         }
         double* eA = nullptr;
@@ -479,7 +481,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             EnergyArgs ea{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles, eA, eB,
                           bitmap, pl.bitmapWords, counter};
             if (e == hipSuccess) {
-                StageTimer t(h, ST_ENERGY);
+                StageTimer t(th, ST_ENERGY);
                 e = launch_energy(ea, s);
             }
         }
@@ -532,7 +534,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.bitmap = bitmap;
         g.shadow = pl.correction && !is_zero(p.beta) ? ws + pl.offShadow : nullptr;
         if (e == hipSuccess) {
-            StageTimer t(h, ST_GEMM);
+            StageTimer t(th, ST_GEMM);
             if (pl.path == 4)
                 e = launch_gemm_lowp_products(g, pl.pieces, pl.variant == 5, s);
             else if (pl.format == FMT_F32)
@@ -566,7 +568,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             mg.corrTau = g.corrTau;
             mg.bitmap = bitmap;
             mg.shadow = g.shadow;
-            StageTimer t(h, ST_MERGE);
+            StageTimer t(th, ST_MERGE);
             e = launch_merge(mg, s);
         }
         if (pl.correction && e == hipSuccess) {
@@ -596,7 +598,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             r.alphaIf = p.alpha.imag;
             r.betaRf = p.beta.real;
             r.betaIf = p.beta.imag;
-            StageTimer t(h, ST_RECOMPUTE);
+            StageTimer t(th, ST_RECOMPUTE);
             e = launch_recompute(r, s);
             h->lastCounter = counter;
             h->lastCounterSlots = 1;
@@ -625,7 +627,7 @@ static void record_plan(aclblasHandle_t h, const Plan& pl)
 aclblasStatus_t run_problem(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
                             int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
                             const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
-                            int ldc, const aclblasCgemm3mConfig* cfg, const char* op)
+                            int ldc, const aclblasCgemm3mConfig* cfg, const char* op, ScratchSource scratch)
 {
     aclblasStatus_t st = validate(handle, ta, tb, m, n, k, alpha, lda, ldb, beta, ldc, op);
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
@@ -650,11 +652,21 @@ aclblasStatus_t run_problem(aclblasHandle_t handle, aclblasOperation_t ta, aclbl
     if (!make_plan(p, cfg, handle->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
     record_plan(handle, pl);
     char* ws = nullptr;
-    if (pl.total > 0) {
+    if (pl.total > 0 && scratch != nullptr) {
+        // the caller's fixed device array instead of the handle's workspace: nothing allocated
+        size_t cap = 0;
+        st = scratch(&ws, &cap, op);
+        if (st != ACLBLAS_STATUS_SUCCESS) return st;
+        if (pl.total > cap) {
+            log_error(op, "this call needs %zu bytes of temporary storage and the fixed array holds %zu", pl.total,
+                      cap);
+            return ACLBLAS_STATUS_NOT_SUPPORTED;
+        }
+    } else if (pl.total > 0) {
         st = acquire_workspace(handle, pl.total, &ws, op);
         if (st != ACLBLAS_STATUS_SUCCESS) return st;
     }
-    return enqueue(handle, p, pl, ws, op);
+    return enqueue(handle, p, pl, ws, op, scratch != nullptr);
 }
 
 }  // namespace c3m

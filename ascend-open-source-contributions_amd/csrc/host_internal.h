@@ -88,7 +88,7 @@ enum Stage { ST_PACK = 0, ST_ENERGY = 1, ST_GEMM = 2, ST_MERGE = 3, ST_RECOMPUTE
 // Records a HIP event pair around one kernel stage when the handle's timing is enabled.
 class StageTimer {
   public:
-    StageTimer(aclblasHandle_t h, int stage);
+    StageTimer(aclblasHandle_t h, int stage);  // h == nullptr: records nothing
     ~StageTimer();
 
   private:
@@ -139,7 +139,8 @@ aclblasStatus_t validate(aclblasHandle_t h, aclblasOperation_t ta, aclblasOperat
 bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan& plan, const char* op);
 aclblasStatus_t ensure_device(aclblasHandle_t h);
 aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, const char* op);
-aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& plan, char* ws, const char* op);
+aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& plan, char* ws, const char* op,
+                        bool noAlloc = false);
 aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op);
 bool is_zero(const aclblasComplex& z);
 
@@ -147,11 +148,18 @@ bool is_zero(const aclblasComplex& z);
 // (exact ops and sizes), else for its (log2 m*n, log2 k) bin.
 bool tuned_lookup(int opA, int opB, long long m, long long n, long long k, bool grouped, int* tile, int* parts);
 
-// One aclblasCgemm3m(Ex) call: validation, plan, workspace, launches (cgemm3m_host.cpp).
+// Temporary storage a call may use instead of the handle's workspace: a fixed device array
+// of the current device (base, capacity in bytes). Used by the no-allocation entry.
+typedef aclblasStatus_t (*ScratchSource)(char** base, size_t* bytes, const char* op);
+
+// One aclblasCgemm3m(Ex) call: validation, plan, workspace, launches (cgemm3m_host.cpp). With a
+// scratch source the call allocates nothing: its temporary storage comes from that array (a
+// plan needing more returns ACLBLAS_STATUS_NOT_SUPPORTED).
 aclblasStatus_t run_problem(aclblasHandle_t handle, aclblasOperation_t ta, aclblasOperation_t tb, int m, int n,
                             int k, const aclblasComplex* alpha, const aclblasComplex* A, int lda,
                             const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
-                            int ldc, const aclblasCgemm3mConfig* cfg, const char* op);
+                            int ldc, const aclblasCgemm3mConfig* cfg, const char* op,
+                            ScratchSource scratch = nullptr);
 // One grouped batch of FP32 problems (grouped_host.cpp); over: optional plan override.
 aclblasStatus_t grouped_run(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
                             const aclblasOperation_t* transb, const int* m, const int* n, const int* k,

@@ -228,6 +228,19 @@ aclblasStatus_t aclblasCgemm3mSetTiming(aclblasHandle_t handle, int enable);
  * ms[5] beta scale; *calls = number of calls. Synchronises the recorded events. */
 aclblasStatus_t aclblasCgemm3mGetTiming(aclblasHandle_t handle, double* ms, long long* calls);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory during
+ * the call (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): aclblasCgemm3m with the same arguments, statuses, plan and results, whose
+ * temporary storage is a fixed 64 MiB device array per GPU (held from the first such call
+ * until the process ends) instead of the handle's workspace. It calls no allocator; a call
+ * that needs more than the array holds returns ACLBLAS_STATUS_NOT_SUPPORTED and launches
+ * nothing (aclblasCgemm3mGetWorkspaceSize with a null config gives the need). Calls on one GPU
+ * share its array: two calls that overlap on one GPU (two streams, two threads) race. */
+aclblasStatus_t aclblasCgemm3mNoAlloc(aclblasHandle_t handle, aclblasOperation_t transa, aclblasOperation_t transb,
+                                      int m, int n, int k, const aclblasComplex* alpha, const aclblasComplex* A,
+                                      int lda, const aclblasComplex* B, int ldb, const aclblasComplex* beta,
+                                      aclblasComplex* C, int ldc);
+
 /* Route error text to a callback (null restores stderr). Process-wide. */
 aclblasStatus_t aclblasSetLogCallback(aclblasLogCallback callback);
 

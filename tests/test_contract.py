@@ -58,8 +58,13 @@ def h():
     hd.close()
 
 
-def call(h, ta="N", tb="N", m=16, n=16, k=16, alpha=1, A=FAKE, lda=16, B=FAKE, ldb=16, beta=0, C=FAKE, ldc=16):
-    return cgemm3m.cgemm3m(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, check=False)
+def call(h, ta="N", tb="N", m=16, n=16, k=16, alpha=1, A=FAKE, lda=16, B=FAKE, ldb=16, beta=0, C=FAKE, ldc=16,
+         entry="aclblasCgemm3m"):
+    if entry == "aclblasCgemm3m":
+        return cgemm3m.cgemm3m(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, check=False)
+    op = lambda t: _lib.OP[t] if isinstance(t, str) else t
+    return getattr(h.L, entry)(h.h, op(ta), op(tb), m, n, k, _lib.complex_arg(alpha), A, lda, B, ldb,
+                               _lib.complex_arg(beta), C, ldc)
 
 
 def test_null_handle():
@@ -100,8 +105,10 @@ def test_null_handle():
     (dict(tb="C", n=20, k=4, ldb=19, ldc=16), 3),          # C: ldb >= n
     (dict(ta="N", m=0, n=0, lda=0, ldc=1), 3),             # lda >= max(1, m)
 ])
-def test_status_contract(h, kw, expect):
-    assert call(h, **kw) == expect
+@pytest.mark.parametrize("entry", ["aclblasCgemm3m", "aclblasCgemm3mNoAlloc"])
+def test_status_contract(h, kw, expect, entry):
+    # the no-allocation entry keeps aclblasCgemm3m's argument contract
+    assert call(h, entry=entry, **kw) == expect
 
 
 def test_validation_order(h):
