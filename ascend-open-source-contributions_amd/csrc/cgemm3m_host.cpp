@@ -352,7 +352,6 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         if (!add_region((size_t)pl.mPad * 8, total, pl.offEA)) return false;
         if (!add_region((size_t)pl.nPad * 8, total, pl.offEB)) return false;
         if (!add_region((size_t)pl.bitmapWords * 8, total, pl.offBitmap)) return false;
-        if (!is_zero(p.beta) && !add_region(outs * 8, total, pl.offShadow)) return false;
         if (!add_region(256, total, pl.offCounter)) return false;
     }
     if (pl.format == FMT_F16) {
@@ -532,7 +531,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.corrScale = amag * unit;
         g.corrTau = 0x1p-8;
         g.bitmap = bitmap;
-        g.shadow = pl.correction && !is_zero(p.beta) ? ws + pl.offShadow : nullptr;
+        g.src = SourceOperands{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.k};
         if (e == hipSuccess) {
             StageTimer t(th, ST_GEMM);
             if (pl.path == 4)
@@ -567,7 +566,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             mg.corrScale = g.corrScale;
             mg.corrTau = g.corrTau;
             mg.bitmap = bitmap;
-            mg.shadow = g.shadow;
+            mg.src = g.src;
             StageTimer t(th, ST_MERGE);
             e = launch_merge(mg, s);
         }
@@ -589,7 +588,6 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             r.betaZero = g.betaZero;
             r.bitmap = bitmap;
             r.words = pl.bitmapWords;
-            r.shadow = g.shadow;
             r.C = p.C;
             r.ldc = p.ldc;
             r.counter = counter;

@@ -139,13 +139,27 @@ __device__ __forceinline__ float2 apply_scalars(float re, float im, float ar, fl
     return c;
 }
 
-// Rounding-correction screen: flag an output whose FP32 error estimate
-// corrScale * |op(A)_i|_2 * |op(B)_j|_2 exceeds corrTau * |c|, or that is not finite. Both
-// sides are non-negative, so their squares are compared (ea, eb are the squared norms): no
-// square root per output; a NaN or infinite estimate still flags.
+__device__ __forceinline__ bool finite2(float2 c) { return isfinite(c.x) && isfinite(c.y); }
+
+// An output that came out non-finite (NaN or Inf in a component), recomputed from the stored
+// operands in the reference BLAS's FP32 order. The 3M planes can overflow where that order
+// does not (A = (3e38, 3e38) times B = (1, 0): Ar + Ai = Inf, the reference gives
+// (3e38, 3e38)), and that order can overflow where FP64 does not (two terms of +-4e38 give
+// Inf - Inf = NaN), so this keeps an output finite exactly when the reference BLAS's is.
+// Every epilogue applies it, whatever the rounding correction does.
+__device__ __forceinline__ float2 reference_order_output(const SourceOperands& s, long long i, long long j,
+                                                         float2 alpha, float2 beta, bool betaZero, float2 c0)
+{
+    return netlib_output(reinterpret_cast<const float2*>(s.A), s.lda, s.opA, reinterpret_cast<const float2*>(s.B),
+                         s.ldb, s.opB, i, j, s.k, alpha, beta, betaZero, c0);
+}
+
+// Rounding-correction screen for a finite output: flag it when its FP32 error estimate
+// corrScale * |op(A)_i|_2 * |op(B)_j|_2 exceeds corrTau * |c|. Both sides are non-negative,
+// so their squares are compared (ea, eb are the squared norms): no square root per output; a
+// NaN or infinite estimate still flags.
 __device__ __forceinline__ bool needs_recompute(float2 c, double ea, double eb, double corrScale, double corrTau)
 {
-    if (!isfinite(c.x) || !isfinite(c.y)) return true;
     const double mag2 = (double)c.x * c.x + (double)c.y * c.y;
     const double est2 = (corrScale * corrScale) * (ea * eb);
     return !(est2 <= (corrTau * corrTau) * mag2);
@@ -161,11 +175,14 @@ __device__ __forceinline__ void flag_output(unsigned long long* bitmap, long lon
 // iBase + 32*b and, in register r, column jBase + 32*a + 8*(r/4) + (r%4).
 // PARTIAL: write this K partition's (re, im) planes (ld = mPad, padded extent, no checks).
 // Otherwise: alpha/beta in registers, C read only when beta != 0, rows >= m and columns
-// >= n never touched, and the rounding-correction screen on every written output.
+// >= n never touched. An output flagged by the rounding-correction screen is not written
+// (the recompute pass reads C's input there); a non-finite one is recomputed by its lane in
+// the reference BLAS's order once the tile is stored.
 template <int TN, int TM, bool PARTIAL, int NACC = 3>
 __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC][TN][TM], int part, int iBase,
                                            int jBase)
 {
+    static_assert(TN * TM * 16 <= 64, "one bit per output of the lane");
     // NACC == 3: the three products, combined by the 3M identity; NACC == 2: (re, im) as
     // accumulated by the four-product form.
     auto combine = [&](int a, int b, int r, float& re, float& im) {
@@ -200,6 +217,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC
         // cost one full memory round trip per output.
         float2* C = reinterpret_cast<float2*>(g.C);
         const bool bz = g.betaZero != 0;
+        unsigned long long bad = 0ull;  // this lane's non-finite outputs, by (a, b, r)
         double ea[TM];
         if (g.correction) {
 #pragma unroll
@@ -239,14 +257,26 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC
                         combine(a, b, r, re, im);
                         const size_t o = (size_t)j * g.ldc + i;
                         const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0[b][r]);
-                        if (g.correction && needs_recompute(c, ea[b], eb[r], g.corrScale, g.corrTau)) {
-                            const long long lin = (long long)j * g.m + i;
-                            flag_output(g.bitmap, lin);
-                            if (!bz) reinterpret_cast<float2*>(g.shadow)[lin] = c0[b][r];
-                        }
-                        C[o] = c;
+                        if (!finite2(c))
+                            bad |= 1ull << ((a * TM + b) * 16 + r);
+                        else if (g.correction && needs_recompute(c, ea[b], eb[r], g.corrScale, g.corrTau))
+                            flag_output(g.bitmap, (long long)j * g.m + i);
+                        else
+                            C[o] = c;
                     }
                 }
+        }
+        // Non-finite outputs (rare): C still holds the input there.
+        while (bad != 0ull) {
+            const int t = __builtin_ctzll(bad);
+            bad &= bad - 1ull;
+            const int r = t & 15, b = (t >> 4) % TM, a = (t >> 4) / TM;
+            const int i = iBase + b * 32;
+            const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+            const size_t o = (size_t)j * g.ldc + i;
+            const float2 c0 = bz ? make_float2(0.0f, 0.0f) : C[o];
+            C[o] = reference_order_output(g.src, i, j, make_float2(g.alphaR, g.alphaI), make_float2(g.betaR, g.betaI),
+                                          bz, c0);
         }
     }
 }

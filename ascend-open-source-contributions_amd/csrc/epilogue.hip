@@ -8,8 +8,9 @@
 //   scale_c_kernel   alpha == 0 or k == 0 (reference HandleCgemmAlphaZero,
 //                    cgemm_host.cpp:636-663): C = beta*C, or zeros when beta == 0.
 //   recompute_kernel rounding correction (reference K5-K8, cgemm3m_rounding_kernel.cpp:
-//                    132-497, re-designed): every output flagged by the GEMM epilogue is
-//                    recomputed from the original complex64 inputs in FP64, each lane
+//                    132-497, re-designed): every output flagged by the GEMM epilogue (which
+//                    left C's input in place there) is recomputed from the original
+//                    complex64 inputs in FP64, each lane
 //                    taking a fixed residue class of k in increasing order and the 64 lane
 //                    sums combined by a fixed butterfly, then alpha/beta applied in FP64
 //                    and the result rounded once to FP32.
@@ -44,10 +45,14 @@ __device__ __forceinline__ void merge_one(const MergeArgs& a, int i, int j)
     float2 c0 = make_float2(0.0f, 0.0f);
     if (!bz) c0 = C[co];
     const float2 c = apply_scalars(re, im, a.alphaR, a.alphaI, a.betaR, a.betaI, bz, c0);
+    if (!finite2(c)) {
+        C[co] = reference_order_output(a.src, i, j, make_float2(a.alphaR, a.alphaI), make_float2(a.betaR, a.betaI), bz,
+                                       c0);
+        return;
+    }
     if (a.correction && needs_recompute(c, a.eA[i], a.eB[j], a.corrScale, a.corrTau)) {
-        const long long lin = (long long)j * a.m + i;
-        flag_output(a.bitmap, lin);
-        if (!bz) reinterpret_cast<float2*>(a.shadow)[lin] = c0;
+        flag_output(a.bitmap, (long long)j * a.m + i);  // C keeps its input for the recompute
+        return;
     }
     C[co] = c;
 }
@@ -102,17 +107,22 @@ __device__ __forceinline__ void merge_four(const MergeArgs& a, int i0, int j)
 #pragma unroll
         for (int u = 0; u < 4; ++u) c0[u] = (!bz && u < live) ? C[co + u] : make_float2(0.0f, 0.0f);
     }
+    // A flagged output gets its input back (what C held, or anything when beta == 0, where the
+    // recompute does not read it); a non-finite one is recomputed here in the reference
+    // BLAS's order.
     float2 out[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         if (u >= live) break;
         const float2 c = apply_scalars(re[u], im[u], a.alphaR, a.alphaI, a.betaR, a.betaI, bz, c0[u]);
-        if (a.correction && needs_recompute(c, a.eA[i0 + u], a.eB[j], a.corrScale, a.corrTau)) {
-            const long long lin = (long long)j * a.m + i0 + u;
-            flag_output(a.bitmap, lin);
-            if (!bz) reinterpret_cast<float2*>(a.shadow)[lin] = c0[u];
-        }
         out[u] = c;
+        if (!finite2(c)) {
+            out[u] = reference_order_output(a.src, i0 + u, j, make_float2(a.alphaR, a.alphaI),
+                                            make_float2(a.betaR, a.betaI), bz, c0[u]);
+        } else if (a.correction && needs_recompute(c, a.eA[i0 + u], a.eB[j], a.corrScale, a.corrTau)) {
+            flag_output(a.bitmap, (long long)j * a.m + i0 + u);
+            out[u] = c0[u];
+        }
     }
     if (wide) {
         reinterpret_cast<f4*>(C + co)[0] = f4{out[0].x, out[0].y, out[1].x, out[1].y};
@@ -197,7 +207,7 @@ __device__ __forceinline__ double2 load_op(const float2* X, long long ld, int op
     return make_double2((double)v.x, op == 113 ? -(double)v.y : (double)v.y);
 }
 
-__global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
+__device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
 {
     const int lane = threadIdx.x & 63;
     const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
@@ -220,8 +230,7 @@ __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
                 if ((bits >> lane) & 1ull) {
                     const long long lin = w * 64 + lane;
                     const long long i = lin % a.m, j = lin / a.m;
-                    const float2 c0 =
-                        a.betaZero ? make_float2(0.0f, 0.0f) : reinterpret_cast<const float2*>(a.shadow)[lin];
+                    const float2 c0 = a.betaZero ? make_float2(0.0f, 0.0f) : C[(size_t)j * a.ldc + i];
                     C[(size_t)j * a.ldc + i] = netlib_output(A, a.lda, a.opA, B, a.ldb, a.opB, i, j, a.k,
                                                              make_float2(a.alphaRf, a.alphaIf),
                                                              make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
@@ -270,7 +279,7 @@ __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
                     double cr = a.alphaR * sr - a.alphaI * si;
                     double ci = a.alphaR * si + a.alphaI * sr;
                     if (!a.betaZero) {
-                        const float2 c0 = reinterpret_cast<const float2*>(a.shadow)[lin];
+                        const float2 c0 = C[(size_t)j * a.ldc + i];  // the epilogue left the input there
                         cr += a.betaR * c0.x - a.betaI * c0.y;
                         ci += a.betaR * c0.y + a.betaI * c0.x;
                     }
@@ -280,6 +289,25 @@ __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a)
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a) { recompute_body(a); }
+
+// Grouped batch: blockIdx.y picks the member (one with the rounding correction on).
+__global__ __launch_bounds__(256) void recompute_grouped_kernel(const RecomputeArgs* items)
+{
+    const RecomputeArgs a = items[blockIdx.y];
+    recompute_body(a);
+}
+
+hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s)
+{
+    long long bx = (maxWords + 255) / 256;  // 4 waves per block, 64 bitmap words per wave
+    if (bx > 1024) bx = 1024;
+    if (bx < 1) bx = 1;
+    if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(recompute_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items);
+    return hipGetLastError();
 }
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)

@@ -31,13 +31,19 @@
 // fails the screen (as in the other paths) is recomputed by the whole workgroup in FP64 from
 // the inputs (thread t takes k = t, t+256, ...; fixed-order reduction), alpha/beta applied in
 // FP64, rounded once. Flagged outputs are recomputed one after another in whatever order the
-// workgroup listed them; each one's arithmetic is fixed, so the result is deterministic.
+// workgroup listed them; each one's arithmetic is fixed, so the result is deterministic. An
+// output that comes out non-finite is listed too, whether or not the correction is on, and
+// recomputed by one thread in the reference BLAS's sequential FP32 order (finite exactly when
+// that order's result is).
 #include "device_util.h"
 #include "launch.h"
 
 namespace c3m {
 
 namespace {
+
+// Marks a listed output (tile-local index 0..255) as non-finite rather than screen-flagged.
+constexpr int kNonFinite = 0x100;
 
 // 16-deep k chunks per register set. One: 58 VGPRs (six waves per SIMD) against 93 for two
 // (four waves); measured faster on most small shapes (581 x 782 x 175: 18.9 -> 16.1 us,
@@ -182,8 +188,8 @@ __device__ __forceinline__ void small_tile(const SmallArgs& g, int tilesM, int t
             ered[wave][0][lane] = ea4;
             ered[wave][1][lane] = eb4;
         }
-        if (tid == 0) flagged = 0;
     }
+    if (tid == 0) flagged = 0;
     __syncthreads();
     // thread t < 32: the tile's energy of row t (t < 16) or column t - 16, waves in order
     if (g.correction && tid < 32) {
@@ -260,32 +266,41 @@ __device__ __forceinline__ void small_tile(const SmallArgs& g, int tilesM, int t
         float2 c0 = make_float2(0.0f, 0.0f);
         if (!bz) c0 = C[co];
         const float2 c = apply_scalars(re, im, g.alphaR, g.alphaI, g.betaR, g.betaI, bz, c0);
-        const bool flag = g.correction && needs_recompute(c, eTot[0][o & 15], eTot[1][o >> 4], g.corrScale, g.corrTau);
+        // non-finite outputs (listed with kNonFinite) and outputs the screen flags are
+        // recomputed below; the rest are stored now
+        const bool bad = !finite2(c);
+        const bool flag =
+            bad || (g.correction && needs_recompute(c, eTot[0][o & 15], eTot[1][o >> 4], g.corrScale, g.corrTau));
         if (flag) {
             const int slot = atomicAdd(&flagged, 1);
-            flagList[slot] = o;
+            flagList[slot] = o | (bad ? kNonFinite : 0);
             flagC0[slot] = c0;
         } else {
             C[co] = c;
         }
     }
-    if (!g.correction) return;
     __syncthreads();
     const int count = flagged;
-    if (g.correction == 2) {
-        // the reference BLAS's sequential FP32 order: one flagged output per thread
-        for (int e = tid; e < count; e += NT) {
-            const int ft = flagList[e];
-            const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);
-            C[(size_t)fj * g.ldc + fi] = netlib_output(A, g.lda, g.opA, B, g.ldb, g.opB, fi, fj, K,
-                                                       make_float2(g.alphaR, g.alphaI), make_float2(g.betaR, g.betaI),
-                                                       bz, flagC0[e]);
-        }
-        if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = (unsigned int)count;
+    if (count == 0) {
+        if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = 0u;
         return;
     }
+    // the reference BLAS's sequential FP32 order, one output per thread: every non-finite
+    // output, and every flagged one under correction 2
+    for (int e = tid; e < count; e += NT) {
+        const int ft = flagList[e];
+        if ((ft & kNonFinite) == 0 && g.correction != 2) continue;
+        const long long fi = i0 + (ft & 255 & 15), fj = j0 + ((ft & 255) >> 4);
+        C[(size_t)fj * g.ldc + fi] = netlib_output(A, g.lda, g.opA, B, g.ldb, g.opB, fi, fj, K,
+                                                   make_float2(g.alphaR, g.alphaI), make_float2(g.betaR, g.betaI), bz,
+                                                   flagC0[e]);
+    }
+    if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = (unsigned int)count;
+    if (g.correction != 1) return;
+    // FP64, the whole workgroup on one flagged output at a time
     for (int e = 0; e < count; ++e) {
         const int ft = flagList[e];
+        if (ft & kNonFinite) continue;  // the same for every thread
         const long long fi = i0 + (ft & 15), fj = j0 + (ft >> 4);
         double sr = 0.0, si = 0.0;
         for (long long l = tid; l < K; l += NT) {
@@ -325,7 +340,6 @@ __device__ __forceinline__ void small_tile(const SmallArgs& g, int tilesM, int t
         }
         __syncthreads();
     }
-    if (tid == 0 && g.flagCount != nullptr) g.flagCount[tile] = (unsigned int)count;
 }
 
 template <int W>

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -146,6 +147,7 @@ struct GroupedLayout {
     std::vector<Plan> smallPlans;
     size_t descBytes = 0, offQueue = 0, total = 0;
     size_t nProb = 0, nPack = 0, nQuad = 0, nMerge = 0;
+    size_t nCorr = 0;               // packed members with the rounding correction on
     long long packBlocks = 0;
     int maxMergeOutputs = 0;
 };
@@ -154,13 +156,14 @@ static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // Plan of one grouped member: the tuned table's plan for grouped members (tile 64 / 128 with
 // K partitions, or the one-launch kernel, tile 32 / 16, unsliced), else 128 tiles for members
-// of at least 256 x 256 and 64 tiles below, no K split; no rounding correction; the caller's
-// tile/kSplit/identity override when given (variant 5: the one-launch kernel). The one-problem
+// of at least 256 x 256 and 64 tiles below, no K split; the rounding correction of a single
+// aclblasCgemm3m call (on when k >= 1024); the caller's
+// tile/kSplit/identity/correction override when given (variant 5: the one-launch kernel). The one-problem
 // form of the same plan (variant: LDS-free kernel for 128 tiles, as the grouped kernel uses)
 // gives byte-identical results, which the strided-batched entry relies on.
 aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over)
 {
-    aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, -1, 0, 1};
+    aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, 0, 0, 1};
     int tt = 0, tp = 0;
     bool small = false;
     if (over != nullptr && over->variant == 5) {
@@ -177,6 +180,7 @@ aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCge
         cfg.tile = (m >= 256 && n >= 256) ? 128 : 64;
     }
     if (over != nullptr && over->identity != 0) cfg.identity = over->identity;
+    if (over != nullptr && over->correction != 0) cfg.correction = over->correction;
     if (small) {  // four waves per tile (table tile 32) or one (tile 16); no k slices
         cfg.variant = 5;
         cfg.tile = tt == 16 ? 16 : 0;
@@ -238,13 +242,15 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             ++L.nMerge;
             L.maxMergeOutputs = std::max<long long>(L.maxMergeOutputs, (long long)L.probs[i].m * L.probs[i].n);
         }
+        if (pl.correction) ++L.nCorr;
     }
     L.nQuad += (loose + 3) / 4;
     L.nProb = L.probs.size();
     L.nPack = 2 * L.nProb;
     L.descBytes = align256(L.nProb * sizeof(GroupProblem)) + align256(L.nPack * sizeof(GroupPack)) +
                   align256(4 * L.nQuad * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge)) +
-                  align256(L.small.size() * sizeof(GroupSmall));
+                  align256(L.small.size() * sizeof(GroupSmall)) + align256(L.nCorr * sizeof(EnergyArgs)) +
+                  align256(L.nCorr * sizeof(RecomputeArgs));
     L.offQueue = total;
     total += 256;
     L.total = L.nProb ? total : 0;
@@ -342,6 +348,10 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         std::vector<Quad> quads;
         std::vector<Cost> loose;
         quads.reserve(L.nQuad);
+        // members with the rounding correction: energies (after the pack) and recompute (last)
+        std::vector<EnergyArgs> energies;
+        std::vector<RecomputeArgs> recomputes;
+        long long maxEnergyWork = 0, maxWords = 0;
         long long packBlocks = 0;
         size_t nm = 0;
         for (size_t i = 0; i < L.nProb; ++i) {
@@ -369,7 +379,8 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 a.nplanes = 3;  // grouped members use a 3M identity
                 a.planes = pws + (isA ? pl.offA : pl.offB);
                 a.planeBytes = isA ? pl.aPlaneBytes : pl.bPlaneBytes;
-                a.energyPartial = nullptr;
+                a.energyPartial =
+                    pl.correction ? reinterpret_cast<double*>(pws + (isA ? pl.offEPA : pl.offEPB)) : nullptr;
                 a.scaleExp = nullptr;
                 pk[2 * i + role].a = a;
                 pk[2 * i + role].firstBlock = packBlocks;
@@ -399,6 +410,49 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             g.ldc = p.ldc;
             g.partial = pl.parts > 1 ? reinterpret_cast<float*>(pws + pl.offPartial) : nullptr;
             g.partStride = pl.partStride;
+            g.src = SourceOperands{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.k};
+            // the one-problem path's rounding correction (FP32 products: unit 2^-24)
+            g.correction = pl.correction;
+            if (pl.correction) {
+                const double amag =
+                    std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
+                g.eA = reinterpret_cast<const double*>(pws + pl.offEA);
+                g.eB = reinterpret_cast<const double*>(pws + pl.offEB);
+                g.corrScale = amag * 0x1p-24;
+                g.corrTau = 0x1p-8;
+                g.bitmap = reinterpret_cast<unsigned long long*>(pws + pl.offBitmap);
+                energies.push_back(EnergyArgs{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles,
+                                              const_cast<double*>(g.eA), const_cast<double*>(g.eB), g.bitmap,
+                                              pl.bitmapWords, nullptr});
+                maxEnergyWork = std::max<long long>({maxEnergyWork, pl.mPad, pl.nPad, pl.bitmapWords});
+                RecomputeArgs r{};
+                r.A = p.A;
+                r.lda = p.lda;
+                r.opA = (int)p.ta;
+                r.B = p.B;
+                r.ldb = p.ldb;
+                r.opB = (int)p.tb;
+                r.m = p.m;
+                r.n = p.n;
+                r.k = p.k;
+                r.alphaR = p.alpha.real;
+                r.alphaI = p.alpha.imag;
+                r.betaR = p.beta.real;
+                r.betaI = p.beta.imag;
+                r.betaZero = g.betaZero;
+                r.bitmap = g.bitmap;
+                r.words = pl.bitmapWords;
+                r.C = p.C;
+                r.ldc = p.ldc;
+                r.counter = nullptr;
+                r.netlib = pl.correction == 2 ? 1 : 0;
+                r.alphaRf = p.alpha.real;
+                r.alphaIf = p.alpha.imag;
+                r.betaRf = p.beta.real;
+                r.betaIf = p.beta.imag;
+                recomputes.push_back(r);
+                maxWords = std::max<long long>(maxWords, pl.bitmapWords);
+            }
             // 64 x 64 blocks: whole 128 x 128 regions as quads in the 128 x 128 kernel's wave
             // order (wave w: block row 2r + w % 2, block column 2c + w / 2); the odd last block
             // row and column go to the loose list
@@ -433,6 +487,13 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 a.betaZero = g.betaZero;
                 a.C = p.C;
                 a.ldc = p.ldc;
+                a.correction = g.correction;
+                a.eA = g.eA;
+                a.eB = g.eB;
+                a.corrScale = g.corrScale;
+                a.corrTau = g.corrTau;
+                a.bitmap = g.bitmap;
+                a.src = g.src;
             }
         }
         // Longest first; ties in (problem, part, tn, tm) order, so the list is a pure function
@@ -474,15 +535,28 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 const Plan& pl = L.smallPlans[i];
                 if ((pl.smallWaves == 1) != (form == 1)) continue;
                 GroupSmall& it = sm[nSmall[0] + nSmall[1]];
+                const double amag =
+                    std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
                 it.a = SmallArgs{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.m, p.n, p.k, pl.identity,
                                  p.alpha.real, p.alpha.imag, p.beta.real, p.beta.imag, is_zero(p.beta) ? 1 : 0,
-                                 p.C, p.ldc, 0, 0.0, 0.0, nullptr, 1, pl.smallWaves, nullptr, nullptr, nullptr};
+                                 p.C, p.ldc, pl.correction, amag * 0x1p-24, 0x1p-8, nullptr, 1, pl.smallWaves,
+                                 nullptr, nullptr, nullptr};
                 it.tilesM = (p.m + 15) / 16;
                 it.tiles = (int)pl.smallTiles;
                 it.firstBlock = smallBlocks[form];
                 smallBlocks[form] += pl.smallTiles;
                 ++nSmall[form];
             }
+        const size_t offEnergy = offSmall + align256(L.small.size() * sizeof(GroupSmall));
+        const size_t offRecompute = offEnergy + align256(L.nCorr * sizeof(EnergyArgs));
+        if (energies.size() != L.nCorr || recomputes.size() != L.nCorr) {
+            log_error(op, "internal error: %zu corrected members laid out, %zu counted", energies.size(), L.nCorr);
+            return ACLBLAS_STATUS_INTERNAL_ERROR;
+        }
+        if (L.nCorr > 0) {
+            std::memcpy(blob.data() + offEnergy, energies.data(), L.nCorr * sizeof(EnergyArgs));
+            std::memcpy(blob.data() + offRecompute, recomputes.data(), L.nCorr * sizeof(RecomputeArgs));
+        }
 
         const char* dDesc = stage_descriptors(handle, blob, s, &e);
         if (dDesc == nullptr) {
@@ -535,6 +609,11 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             StageTimer t(handle, ST_PACK);
             e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);
         }
+        if (e == hipSuccess && L.nCorr > 0) {
+            StageTimer t(handle, ST_ENERGY);
+            e = launch_energy_grouped(reinterpret_cast<const EnergyArgs*>(dDesc + offEnergy), (int)L.nCorr,
+                                      maxEnergyWork, s);
+        }
         if (e == hipSuccess && L.nProb > 0) {
             StageTimer t(handle, ST_GEMM);
             const int cus = handle->cus > 0 ? handle->cus : 256;
@@ -545,6 +624,11 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         if (e == hipSuccess && nm > 0) {
             StageTimer t(handle, ST_MERGE);
             e = launch_merge_grouped(dMerge, (int)nm, L.maxMergeOutputs, s);
+        }
+        if (e == hipSuccess && L.nCorr > 0) {
+            StageTimer t(handle, ST_RECOMPUTE);
+            e = launch_recompute_grouped(reinterpret_cast<const RecomputeArgs*>(dDesc + offRecompute), (int)L.nCorr,
+                                         maxWords, s);
         }
         if (fork && e == hipSuccess) e = hipStreamWaitEvent(s, handle->joinEv, 0);
     }
@@ -591,9 +675,10 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle
     if (config != nullptr &&
         ((small ? (config->tile != 0 && config->tile != 16) || config->kSplit > 1
                 : config->tile != 0 && config->tile != 64 && config->tile != 128) ||
-         config->kSplit < 0 || config->kSplit > 16 || config->identity < 0 || config->identity > ID_STANDARD)) {
-        log_error(op, "invalid grouped config (tile %d, kSplit %d, identity %d, variant %d)", config->tile,
-                  config->kSplit, config->identity, config->variant);
+         config->kSplit < 0 || config->kSplit > 16 || config->identity < 0 || config->identity > ID_STANDARD ||
+         config->correction < -1 || config->correction > 2)) {
+        log_error(op, "invalid grouped config (tile %d, kSplit %d, identity %d, variant %d, correction %d)",
+                  config->tile, config->kSplit, config->identity, config->variant, config->correction);
         return ACLBLAS_STATUS_INVALID_VALUE;
     }
     return grouped_run(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,

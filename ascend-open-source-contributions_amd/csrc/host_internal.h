@@ -116,7 +116,7 @@ struct Plan {
     int planesA = 3, planesB = 3;  // planes * pieces per operand
     // workspace regions (byte offsets)
     size_t offA = 0, offB = 0, offPartial = 0, offEPA = 0, offEPB = 0, offEA = 0, offEB = 0;
-    size_t offBitmap = 0, offShadow = 0, offCounter = 0, offScaleA = 0, offScaleB = 0,
+    size_t offBitmap = 0, offCounter = 0, offScaleA = 0, offScaleB = 0,
            offSumA = 0, offSumB = 0, offArrive = 0;
     size_t partStride = 0;       // floats per partition (re plane + im plane)
     long long bitmapWords = 0;

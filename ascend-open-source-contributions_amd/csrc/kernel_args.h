@@ -81,6 +81,14 @@ struct EnergyArgs {
     unsigned int* counter;
 };
 
+// The caller's operands as stored (interleaved complex64), for recomputing single outputs in
+// the reference BLAS's order (an output the 3M identity left non-finite).
+struct SourceOperands {
+    const void* A; long long lda; int opA;  // op: 111 N, 112 T, 113 C
+    const void* B; long long ldb; int opB;
+    int k;
+};
+
 struct GemmArgs {
     const void* Ap; size_t aPlaneBytes; int mPad;   // packed A planes (role 0)
     const void* Bp; size_t bPlaneBytes; int nPad;   // packed B planes (role 1)
@@ -97,13 +105,14 @@ struct GemmArgs {
     // FP16 block scaling: exponents [3][pad/128][kBlocks128] and their per-row-block summaries
     const int* scaleA; const int* scaleB; int kBlocks128;
     const int* sumA; const int* sumB;
-    // rounding correction (fused path only)
+    // rounding correction (fused path only); a flagged output is left unwritten (C keeps its
+    // input, which the recompute reads when beta != 0)
     int correction;
     const double* eA; const double* eB;
     double corrScale;                       // |alpha| * u  (error estimate = corrScale * |a_i||b_j|)
     double corrTau;                         // flag when estimate > corrTau * |c|
     unsigned long long* bitmap;
-    void* shadow;                           // beta != 0: original C of flagged outputs, ld = m
+    SourceOperands src;                     // non-finite outputs are recomputed from these
 };
 
 struct SmallArgs {      // one-launch path for small problems: complex inputs read directly
@@ -135,7 +144,8 @@ struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/b
     void* C; long long ldc;
     int correction;
     const double* eA; const double* eB; double corrScale, corrTau;
-    unsigned long long* bitmap; void* shadow;
+    unsigned long long* bitmap;
+    SourceOperands src; // non-finite outputs are recomputed from these
     int cWide;          // set by launch_merge: C is 16-byte aligned and ldc is even
 };
 
@@ -149,8 +159,7 @@ struct RecomputeArgs {  // FP64 fixed-order recompute of flagged outputs
     int m, n, k;
     double alphaR, alphaI, betaR, betaI; int betaZero;
     const unsigned long long* bitmap; long long words;
-    const void* shadow;
-    void* C; long long ldc;
+    void* C; long long ldc;             // a flagged output still holds its input (read when beta != 0)
     unsigned int* counter;
     int netlib;             // 1: recompute in the reference BLAS's FP32 order instead of FP64
     float alphaRf, alphaIf, betaRf, betaIf;  // the scalars as given (FP32) for that order
@@ -181,7 +190,7 @@ struct GroupMerge {       // one K-split problem for the grouped merge launch
     MergeArgs a;
 };
 struct GroupSmall {       // one problem of the grouped one-launch (small-problem) kernel
-    SmallArgs a;          // unsliced, no correction
+    SmallArgs a;          // unsliced
     int tilesM, tiles;    // 16x16 output tiles: per column of tiles, in all
     long long firstBlock; // exclusive prefix of workgroups
 };

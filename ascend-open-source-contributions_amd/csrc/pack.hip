@@ -191,8 +191,8 @@ hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blo
 }
 
 // FP16 block scale: exponent e per (plane, 128 mn, 128 k) block with max|x| * 2^-e in
-// [2^14, 2^15), so the block's largest value keeps full FP16 precision without overflow;
-// kScaleNone for a block that holds only zeros.
+// [2^14, 2^15) over the block's finite values, so its largest finite value keeps full FP16
+// precision without overflow; kScaleNone for a block with no finite non-zero value.
 // One 1024-thread workgroup per (128 mn, 128 k) block: 16 elements per thread, loaded four at
 // a time (clamped in-range addresses, out-of-range elements count as zero, no branch around a
 // load), then the maxima reduced within each wave and over the 16 waves (max is order-free).
@@ -219,10 +219,16 @@ __global__ __launch_bounds__(1024) void scale_exp_kernel(ScaleArgs a0, ScaleArgs
             v[t].x = in ? v[t].x : 0.0f;
             v[t].y = in ? (a.conj ? -v[t].y : v[t].y) : 0.0f;
         }
+        // The finite maximum only: an Inf or NaN (an input, or a plane sum that overflows)
+        // passes through the scaling as itself and must not set the block's scale, which would
+        // leave its finite values unscaled (overflowing FP16 or losing precision).
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) mx[q] = fmaxf(mx[q], fabsf(plane_value(a.role, a.identity, q, v[t].x, v[t].y)));
+            for (int q = 0; q < 3; ++q) {
+                const float x = fabsf(plane_value(a.role, a.identity, q, v[t].x, v[t].y));
+                mx[q] = fmaxf(mx[q], x <= 3.40282347e38f ? x : 0.0f);
+            }
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -328,7 +334,7 @@ __device__ __forceinline__ double ordered_sum(const double* p, size_t ld, int n)
     return s;
 }
 
-__global__ __launch_bounds__(256) void energy_kernel(EnergyArgs a)
+__device__ __forceinline__ void energy_body(const EnergyArgs& a)
 {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long stride = (long long)gridDim.x * 256;
@@ -336,6 +342,25 @@ __global__ __launch_bounds__(256) void energy_kernel(EnergyArgs a)
     for (long long r = t; r < a.mnPadB; r += stride) a.eB[r] = ordered_sum(a.partialB + r, a.mnPadB, a.kTiles);
     for (long long w = t; w < a.bitmapWords; w += stride) a.bitmap[w] = 0ull;
     if (t == 0 && a.counter != nullptr) *a.counter = 0u;
+}
+
+__global__ __launch_bounds__(256) void energy_kernel(EnergyArgs a) { energy_body(a); }
+
+// Grouped batch: blockIdx.y picks the member (one with the rounding correction on).
+__global__ __launch_bounds__(256) void energy_grouped_kernel(const EnergyArgs* items)
+{
+    const EnergyArgs a = items[blockIdx.y];
+    energy_body(a);
+}
+
+hipError_t launch_energy_grouped(const EnergyArgs* items, int nitems, long long maxWork, hipStream_t s)
+{
+    long long bx = (maxWork + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    if (bx < 1) bx = 1;
+    if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(energy_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items);
+    return hipGetLastError();
 }
 
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s)

@@ -128,7 +128,10 @@ aclblasStatus_t aclblasCgemm3mStridedBatched(aclblasHandle_t handle, aclblasOper
 typedef enum {
     ACLBLAS_CGEMM3M_FP32 = 0,        /* FP32 MFMA, FP32 accumulation (+ FP64 correction pass) */
     ACLBLAS_CGEMM3M_BF16X3 = 1,      /* each FP32 plane split hi+lo BF16; hi*hi + hi*lo + lo*hi */
-    ACLBLAS_CGEMM3M_FP16_SCALED = 2, /* FP16 planes with a power-of-two scale per 128x128 block */
+    ACLBLAS_CGEMM3M_FP16_SCALED = 2, /* FP16 planes with a power-of-two scale per 128x128 block. Bounds
+                                        the NORMWISE error only (about 5e-4 of max|C| at 8192^3 on
+                                        uniform data): an output much smaller than the largest of its
+                                        block can carry a large relative error (up to 0.66 seen) */
     ACLBLAS_CGEMM3M_BF16X6 = 3       /* three-piece BF16 split, six products (more accurate) */
 } aclblasCgemm3mPrecision_t;
 
@@ -182,7 +185,8 @@ aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count, 
                                              int precision);
 
 /* For the autotuner and callers that pin plans: aclblasCgemm3mGroupedBatched with one
- * config applied to every member (precision; tile and kSplit when nonzero; identity). */
+ * config applied to every member (precision; tile and kSplit when nonzero; identity;
+ * correction when nonzero, else on for members with k >= 1024 as in aclblasCgemm3m). */
 aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
                                                const aclblasOperation_t* transb, const int* m, const int* n,
                                                const int* k, const aclblasComplex* alpha,
