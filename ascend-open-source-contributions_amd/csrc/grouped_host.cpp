@@ -78,63 +78,118 @@ static uint64_t block_hash(const void* data, size_t bytes)
     return h;
 }
 
-// Device copy of a descriptor block. The first call with a given content copies it from a
-// pinned host buffer on the stream; later calls with the same content reuse the device copy
-// without any copy, so a hipGraph captured after one eager call holds kernels only. If the
-// first call is itself being captured, the copy becomes a graph node and a later eager call
-// uploads again. A block used while capturing is marked and never evicted; otherwise, when
-// more than kMaxStaged blocks are held, the least recently used one is freed after the
-// device has finished the work queued so far (eviction never happens during a capture).
-static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>& blob, hipStream_t s,
+// Buffer size of a descriptor slot: a power of two of at least 4 KiB, so that a reused slot
+// rarely needs larger buffers.
+static size_t slot_capacity(size_t bytes)
+{
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+static bool slot_buffers(_aclblas_handle::Staged& st, size_t cap)
+{
+    RelaxedCapture relaxed;
+    st.host = st.device = nullptr;
+    st.capacity = 0;
+    if (hipHostMalloc(&st.host, cap, hipHostMallocDefault) != hipSuccess) {
+        st.host = nullptr;
+        return false;
+    }
+    if (hipMalloc(&st.device, cap) != hipSuccess) {
+        (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.device = nullptr;
+        return false;
+    }
+    st.capacity = cap;
+    return true;
+}
+
+// Device copy of a descriptor block. The first call with a given content copies it from the
+// slot's pinned host buffer on the stream; later calls with the same content reuse the device
+// copy without any copy, so a hipGraph captured after one eager call holds kernels only. If
+// the first call is itself being captured, the copy becomes a graph node and a later eager
+// call uploads again. A slot used while capturing is marked and never reused. A new content
+// takes a new slot until kMaxStaged slots exist, then the least recently used one, after the
+// host has waited for the event recorded after that slot's last kernels (one earlier call of
+// this handle, long finished in a steady loop): no device-wide synchronisation and no
+// allocation per call. *slot receives the slot index (for grouped_run to record its event).
+static const char* stage_descriptors(aclblasHandle_t h, const std::vector<char>& blob, hipStream_t s, size_t* slot,
                                      hipError_t* err)
 {
     const bool capturing = stream_capturing(s);
     const uint64_t key = block_hash(blob.data(), blob.size());
-    size_t idx = h->staged.size();
+    const size_t none = h->staged.size();
+    size_t idx = none;
     for (size_t i = 0; i < h->staged.size(); ++i) {
         const auto& st = h->staged[i];
         if (st.hash == key && st.bytes == blob.size() && std::memcmp(st.host, blob.data(), blob.size()) == 0) idx = i;
     }
-    if (idx == h->staged.size()) {
-        if (!capturing) {
-            size_t loose = 0;
-            for (const auto& st : h->staged) loose += st.captured ? 0 : 1;
-            if (loose >= kMaxStaged) {
-                size_t victim = h->staged.size();
-                for (size_t i = 0; i < h->staged.size(); ++i)
-                    if (!h->staged[i].captured &&
-                        (victim == h->staged.size() || h->staged[i].lastUse < h->staged[victim].lastUse))
-                        victim = i;
-                *err = hipDeviceSynchronize();
+    if (idx == none) {
+        size_t loose = 0, victim = none;
+        for (size_t i = 0; i < h->staged.size(); ++i) {
+            if (h->staged[i].captured) continue;
+            ++loose;
+            if (victim == none || h->staged[i].lastUse < h->staged[victim].lastUse) victim = i;
+        }
+        if (!capturing && loose >= kMaxStaged) {
+            _aclblas_handle::Staged& v = h->staged[victim];
+            if (v.recorded) {
+                *err = hipEventSynchronize(v.done);
                 if (*err != hipSuccess) return nullptr;
-                (void)hipHostFree(h->staged[victim].host);
-                (void)hipFree(h->staged[victim].device);
-                h->staged.erase(h->staged.begin() + (std::ptrdiff_t)victim);
             }
+            if (v.capacity < blob.size()) {  // its buffers are idle now; replace them with larger ones
+                (void)hipHostFree(v.host);
+                (void)hipFree(v.device);
+                if (!slot_buffers(v, slot_capacity(blob.size()))) {
+                    *err = hipErrorOutOfMemory;
+                    (void)hipEventDestroy(v.done);
+                    h->staged.erase(h->staged.begin() + (std::ptrdiff_t)victim);
+                    return nullptr;
+                }
+            }
+            idx = victim;
+        } else {
+            _aclblas_handle::Staged st{};
+            if (!slot_buffers(st, slot_capacity(blob.size()))) {
+                *err = hipErrorOutOfMemory;
+                return nullptr;
+            }
+            RelaxedCapture relaxed;
+            if (hipEventCreateWithFlags(&st.done, hipEventDisableTiming) != hipSuccess) {
+                *err = hipGetLastError();
+                (void)hipHostFree(st.host);
+                (void)hipFree(st.device);
+                return nullptr;
+            }
+            h->staged.push_back(st);
+            idx = h->staged.size() - 1;
         }
-        _aclblas_handle::Staged st{key, blob.size(), nullptr, nullptr, false, false, 0};
-        RelaxedCapture relaxed;
-        if (hipHostMalloc(&st.host, blob.size(), hipHostMallocDefault) != hipSuccess) {
-            *err = hipGetLastError();
-            return nullptr;
-        }
-        if (hipMalloc(&st.device, blob.size()) != hipSuccess) {
-            *err = hipGetLastError();
-            (void)hipHostFree(st.host);
-            return nullptr;
-        }
-        std::memcpy(st.host, blob.data(), blob.size());
-        h->staged.push_back(st);
-        idx = h->staged.size() - 1;
+        _aclblas_handle::Staged& fresh = h->staged[idx];
+        fresh.hash = key;
+        fresh.bytes = blob.size();
+        fresh.uploaded = false;
+        fresh.recorded = false;
+        fresh.doneStream = nullptr;
+        std::memcpy(fresh.host, blob.data(), blob.size());
     }
     _aclblas_handle::Staged& hit = h->staged[idx];
     hit.lastUse = ++h->stageClock;
     if (capturing) hit.captured = true;
+    // A slot last read on another stream: that read (and the upload before it) finishes
+    // before this stream goes on. Not while capturing: a capture may not wait on an event
+    // recorded outside it, and the slot, now marked captured, is never rewritten.
+    if (hit.recorded && hit.doneStream != s && !capturing) {
+        *err = hipStreamWaitEvent(s, hit.done, 0);
+        if (*err != hipSuccess) return nullptr;
+    }
     if (!hit.uploaded) {
         *err = hipMemcpyAsync(hit.device, hit.host, hit.bytes, hipMemcpyHostToDevice, s);
         if (*err != hipSuccess) return nullptr;
         if (!capturing) hit.uploaded = true;
     }
+    *slot = idx;
     return static_cast<const char*>(hit.device);
 }
 
@@ -558,7 +613,8 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             std::memcpy(blob.data() + offRecompute, recomputes.data(), L.nCorr * sizeof(RecomputeArgs));
         }
 
-        const char* dDesc = stage_descriptors(handle, blob, s, &e);
+        size_t slot = 0;
+        const char* dDesc = stage_descriptors(handle, blob, s, &slot, &e);
         if (dDesc == nullptr) {
             log_error(op, "descriptor staging failed: %s", hipGetErrorString(e));
             return ACLBLAS_STATUS_ALLOC_FAILED;
@@ -631,6 +687,14 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                                          maxWords, s);
         }
         if (fork && e == hipSuccess) e = hipStreamWaitEvent(s, handle->joinEv, 0);
+        // the slot is free for reuse once this call's kernels are done (a captured slot is
+        // never reused)
+        _aclblas_handle::Staged& used = handle->staged[slot];
+        if (e == hipSuccess && !used.captured) {
+            e = hipEventRecord(used.done, s);
+            used.doneStream = s;
+            used.recorded = e == hipSuccess;
+        }
     }
     for (const Problem& p : L.scaleOnly) {
         if (e != hipSuccess) break;

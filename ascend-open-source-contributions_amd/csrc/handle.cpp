@@ -115,12 +115,35 @@ aclblasStatus_t acquire_workspace(aclblasHandle_t h, size_t bytes, char** base, 
 
 using namespace c3m;
 
+// cann_ops_blas.h:32-46: the handle comes with its 32 MiB default workspace, reserved here on
+// the current device (ALLOC_FAILED if that fails). On a host without a HIP device the handle is
+// still created, for argument validation, and the workspace is reserved by the first call that
+// needs one.
 extern "C" aclblasStatus_t aclblasCreate(aclblasHandle_t* handle)
 {
     if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     if (*handle != nullptr) return ACLBLAS_STATUS_INVALID_VALUE;
     _aclblas_handle* h = new (std::nothrow) _aclblas_handle();
     if (h == nullptr) return ACLBLAS_STATUS_ALLOC_FAILED;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+    } else if (ensure_device(h) == ACLBLAS_STATUS_SUCCESS) {
+        void* p = nullptr;
+        hipError_t e;
+        {
+            RelaxedCapture relaxed;
+            e = hipMalloc(&p, kDefaultWorkspace);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            log_error("aclblasCreate", "default workspace of %zu bytes could not be reserved", kDefaultWorkspace);
+            delete h;
+            return ACLBLAS_STATUS_ALLOC_FAILED;
+        }
+        h->defWs = p;
+        h->defWsBytes = kDefaultWorkspace;
+    }
     *handle = h;
     return ACLBLAS_STATUS_SUCCESS;
 }
@@ -137,6 +160,7 @@ extern "C" aclblasStatus_t aclblasDestroy(aclblasHandle_t handle)
         for (auto& sb : handle->staged) {
             (void)hipHostFree(sb.host);
             (void)hipFree(sb.device);
+            if (sb.done != nullptr) (void)hipEventDestroy(sb.done);
         }
         for (auto& ev : handle->timed) {
             (void)hipEventDestroy(ev.start);

@@ -32,17 +32,23 @@ struct _aclblas_handle {
     long long timedCalls = 0;
     // Grouped-batch descriptor blocks, keyed by content: a pinned host copy and a device
     // copy that is uploaded once. Later calls with the same problems (and hipGraph captures
-    // of them) only read the device copy. At most kMaxStaged blocks that no capture used are
-    // kept (least recently used evicted); blocks a capture used live until aclblasDestroy,
-    // since a graph may replay them at any time.
+    // of them) only read the device copy. At most kMaxStaged slots that no capture used are
+    // kept; a call with new content reuses the least recently used slot (and its buffers,
+    // when large enough) once the event recorded after that slot's last use has completed,
+    // so the host waits at most for that one earlier call and nothing is allocated per call.
+    // Slots a capture used live until aclblasDestroy, since a graph may replay them.
     struct Staged {
         uint64_t hash;
         size_t bytes;
+        size_t capacity;          // bytes both buffers hold
         void* host;
         void* device;
         bool uploaded;
         bool captured;
         uint64_t lastUse;
+        hipEvent_t done;          // recorded after the last kernel that read the slot
+        hipStream_t doneStream;   // the stream `done` was recorded on (null: never recorded)
+        bool recorded;
     };
     std::vector<Staged> staged;
     uint64_t stageClock = 0;

@@ -67,9 +67,10 @@ typedef void* aclrtStream;
 /* ops-blas/include/cann_ops_blas.h:23: log sink for error text. */
 typedef void (*aclblasLogCallback)(const char*);
 
-/* ops-blas/include/cann_ops_blas.h:32-45. *handle must be nullptr on entry.
- * The 32 MiB default workspace is reserved on first use rather than here, so a handle
- * can be created (and argument validation exercised) on a host without a GPU. */
+/* ops-blas/include/cann_ops_blas.h:32-45. *handle must be nullptr on entry. Reserves the
+ * 32 MiB default workspace on the current device (ACLBLAS_STATUS_ALLOC_FAILED if it cannot).
+ * On a host without a HIP device the handle is still created (argument validation works)
+ * and the workspace is reserved by the first call that needs one. */
 aclblasStatus_t aclblasCreate(aclblasHandle_t* handle);
 
 /* ops-blas/include/cann_ops_blas.h:47-58. Synchronises the stream, frees the default

@@ -313,3 +313,65 @@ def test_single_call_hipgraph_replay(gpu):
         print("single replay", rep, "differing elements", ndiff)
         assert ndiff == 0
     h.close()
+
+
+def test_grouped_many_distinct_pointer_calls(gpu):
+    """More grouped calls with new buffers than the handle keeps descriptor slots (32): each
+    call's descriptors name new pointers, so slots are reused once their last call is done.
+    Every call must give the bytes of the first call on the same data."""
+    import torch
+
+    import cgemm3m
+
+    shapes = [("N", "N", 96, 80, 128), ("C", "T", 130, 70, 200), ("T", "N", 200, 160, 64)]
+    data = []
+    for i, (ta, tb, m, n, k) in enumerate(shapes):
+        lda, ldb = (m if ta == "N" else k), (k if tb == "N" else n)
+        data.append((ta, tb, m, n, k, lda, ldb) + inputs(ta, tb, m, n, k, lda, ldb, m, 9800 + 3 * i))
+    reference = None
+    for call in range(40):
+        keep, probs = [], []
+        for ta, tb, m, n, k, lda, ldb, A, B, C in data:
+            dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+            keep.append(dC)
+            keep += [dA, dB]
+            probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=complex(0.7, -0.3), A=ptr(dA), lda=lda, B=ptr(dB),
+                              ldb=ldb, beta=complex(0.2, 0.9), C=ptr(dC), ldc=m))
+        cgemm3m.grouped(gpu, probs)
+        torch.cuda.synchronize()
+        outs = [keep[3 * i].cpu().numpy() for i in range(len(data))]
+        if reference is None:
+            reference = outs
+            for (ta, tb, m, n, k, lda, ldb, A, B, C), out in zip(data, outs):
+                gold = oracle.cgemm(ta, tb, m, n, k, complex(0.7, -0.3), A, lda, B, ldb, complex(0.2, 0.9), C, m)
+                ok, worst = oracle.reltol_check(region(out, m, n, m), region(gold, m, n, m))
+                assert ok, worst
+        for i, (o, r) in enumerate(zip(outs, reference)):
+            assert o.tobytes() == r.tobytes(), "call %d member %d differs" % (call, i)
+        del keep
+
+
+def test_create_reserves_default_workspace(gpu):
+    """aclblasCreate reserves the 32 MiB default workspace on the device (cann_ops_blas.h:32-46);
+    a call that fits in it allocates nothing more."""
+    import torch
+
+    import cgemm3m
+
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    h = cgemm3m.Handle()
+    free1 = torch.cuda.mem_get_info()[0]
+    assert free0 - free1 >= 32 << 20, (free0, free1)
+    m = n = k = 512
+    A, B, C = inputs("N", "N", m, n, k, m, k, m, 9900)
+    dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+    cfg = cgemm3m.config(tile=64, k_split=1)
+    assert cgemm3m.workspace_size(h, "N", "N", m, n, k, 1, 0, cfg) <= 32 << 20
+    free2 = torch.cuda.mem_get_info()[0]
+    cgemm3m.cgemm3m(h, "N", "N", m, n, k, 1.0, ptr(dA), m, ptr(dB), k, 0.0, ptr(dC), m, cfg)
+    torch.cuda.synchronize()
+    assert h.last_plan()["path"] == 1
+    free3 = torch.cuda.mem_get_info()[0]
+    assert free3 == free2, (free2, free3)
+    h.close()
