@@ -3,9 +3,10 @@ and writes csrc/tuned_table.h (compiled into the library).
 
 A class is the bin (floor(log2(m*n)), floor(log2(k))). One-problem candidates: output tile 64
 or 128 with 1-16 K partitions, and the one-launch small-problem kernel with four waves per
-16x16 output tile ("tile 32") or one ("tile 16"), with 1-16 k slices per output tile. Grouped classes are timed as one aclblasCgemm3mGroupedBatchedEx call
-of identical problems with the member plan pinned. The table is deterministic input to the
-planner: the same arguments always get the same plan.
+16x16 output tile ("tile 32") or one ("tile 16"), with 1-16 k slices per output tile. Grouped
+classes are timed as one aclblasCgemm3mGroupedBatchedEx call of identical problems with the
+member plan pinned. The table holds classes only (no entry for any single shape) and is
+deterministic input to the planner: the same arguments always get the same plan.
 
 One-problem timing, as the reference measures its performance cases (ops-blas/test/gemm/
 cgemm3m/README.md:95-115): the sum of a call's kernels' device durations from a rocprofv3
@@ -20,13 +21,16 @@ geometric mean of time / (that shape's best time) over the class's shapes, and a
 grouped classes with events. Without a profiler, `python -m cgemm3m.autotune events` times
 every candidate with events around the call (launch gaps included).
 
-The shapes: a few squares and tall/flat problems, plus the reference's 200 performance-suite
-cases (tests/golden/pf_cases.csv, their ops and leading dimensions).
+The shapes: a few squares and tall/flat problems, plus this project's own seeded set of
+generated shapes (`generated_shapes`: every dimension 1..4096, all nine op pairs, some padded
+leading dimensions). The reference's performance suite is not among them: `pf_suite.py` at the
+repository root measures it as a held-out set.
 """
 import argparse
 import json
 import math
 import os
+import random
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -47,6 +51,31 @@ SINGLE_SHAPES = [
 # square members (even log2 m*n classes) and m x 2m members (the odd classes between them)
 GROUPED_SHAPES = [(m, m * r, k) for m in (32, 64, 128, 256, 512) for r in (1, 2) if m * r <= 512
                   for k in (64, 256, 1024, 2048)]
+
+
+def generated_shapes(count=320, seed=20261204):
+    """This project's tuning set: `count` shapes with every dimension in 1..4096 (half drawn
+    log-uniformly, half uniformly, so both the small sizes and the mid sizes are dense), op
+    pairs uniform over the nine, and the stored rows as leading dimensions plus a pad of 1-7
+    for about a quarter of them. Seeded: the same list on every machine."""
+    rng = random.Random(seed)
+
+    def dim():
+        if rng.random() < 0.5:
+            return max(1, min(4096, int(round(2.0 ** rng.uniform(0.0, 12.0)))))
+        return rng.randint(1, 4096)
+
+    def pad():
+        return rng.randint(1, 7) if rng.random() < 0.25 else 0
+
+    out = []
+    for _ in range(count):
+        ta, tb = rng.choice("NTC"), rng.choice("NTC")
+        m, n, k = dim(), dim(), dim()
+        lda = (m if ta == "N" else k) + pad()
+        ldb = (k if tb == "N" else n) + pad()
+        out.append((ta, tb, m, n, k, lda, ldb, m + pad()))
+    return out
 
 
 def bin_of(m, n, k):
@@ -73,16 +102,12 @@ def _config(cgemm3m, tile, parts):
 
 
 def shape_list(which):
-    """[(ta, tb, m, n, k, lda, ldb, ldc)] for 'builtin', 'pf' or 'all'."""
+    """[(ta, tb, m, n, k, lda, ldb, ldc)] for 'builtin', 'gen' or 'all'."""
     out = []
     if which in ("builtin", "all"):
         out += [("N", "N", m, n, k, m, k, m) for (m, n, k) in SINGLE_SHAPES]
-    if which in ("pf", "all"):
-        sys.path.insert(0, os.path.dirname(HERE))
-        from cgemm3m import pf_suite
-
-        out += [(c["transa"], c["transb"], c["m"], c["n"], c["k"], c["lda"], c["ldb"], c["ldc"])
-                for c in pf_suite.load_cases()]
+    if which in ("gen", "all"):
+        out += generated_shapes()
     return out
 
 
@@ -224,21 +249,6 @@ def pick_bins(measured):
     return out
 
 
-def pick_shapes(measured, single):
-    """Per measured shape, its own pick (same rule as `choose`) where it differs from its
-    bin's: [((ta, tb, m, n, k), (tile, parts))], first measurement of a repeated shape kept."""
-    bins = {b: best for (b, _, best) in single}
-    out, seen = [], set()
-    for shape, res in measured:
-        if shape in seen:
-            continue
-        seen.add(shape)
-        best = choose(res)
-        if bins.get(bin_of(*shape[2:5])) != best:
-            out.append((shape, best))
-    return out
-
-
 def read_grouped_entries(path=TABLE):
     """The grouped class entries of an existing table: [((mnLog2, kLog2), 1, (tile, parts))]."""
     import re
@@ -289,19 +299,16 @@ def tune_grouped(h, torch, shapes, reps, count=64):
     return out
 
 
-def write_table(entries, path=TABLE, shapes=()):
+def write_table(entries, path=TABLE):
     seen = {}
     for (mnl, kl), grouped, (tile, parts) in entries:
         seen[(mnl, kl, grouped)] = (tile, parts)
-    ops = {"N": 111, "T": 112, "C": 113}
     lines = [
         "// tuned_table.h -- plan table written by cgemm3m/autotune.py from MI355X measurements.",
-        "// kTunedTable key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and",
-        "// K partitions (tile 32 / 16: the one-launch small-problem kernel, four waves / one wave per",
-        "// 16x16 tile; parts = k slices). Bins not listed fall back to the planner's occupancy rule.",
-        "// kTunedShapes: one-problem shapes the tuner measured whose own best plan differs from their",
-        "// bin's, looked up first by exact (opA, opB, m, n, k); a packed-tile entry (tile 64 / 128)",
-        "// also serves the same (m, n, k) under other ops.",
+        "// Key: (floor(log2(m*n)), floor(log2(k)), grouped); value: output tile edge and K partitions",
+        "// (tile 32 / 16: the one-launch small-problem kernel, four waves / one wave per 16x16 tile;",
+        "// parts = k slices). Classes only, tuned on the tuner's own generated shapes; bins not",
+        "// listed fall back to the planner's occupancy rule.",
         "#pragma once",
         "",
         "namespace c3m {",
@@ -310,19 +317,11 @@ def write_table(entries, path=TABLE, shapes=()):
         "    short mnLog2, kLog2, grouped, tile, parts;",
         "};",
         "",
-        "struct TunedShape {",
-        "    int m, n, k;",
-        "    short opA, opB, tile, parts;",
-        "};",
-        "",
         "static const TunedEntry kTunedTable[] = {",
     ]
     for (mnl, kl, g), (tile, parts) in sorted(seen.items()):
         lines.append("    {%d, %d, %d, %d, %d}," % (mnl, kl, g, tile, parts))
-    lines += ["    {0, 0, 0, 0, 0},  // sentinel", "};", "", "static const TunedShape kTunedShapes[] = {"]
-    for (ta, tb, m, n, k), (tile, parts) in sorted(shapes, key=lambda e: (e[0][2], e[0][3], e[0][4], e[0][0], e[0][1])):
-        lines.append("    {%d, %d, %d, %d, %d, %d, %d}," % (m, n, k, ops[ta], ops[tb], tile, parts))
-    lines += ["    {0, 0, 0, 0, 0, 0, 0},  // sentinel", "};", "", "}  // namespace c3m", ""]
+    lines += ["    {0, 0, 0, 0, 0},  // sentinel", "};", "", "}  // namespace c3m", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))
     return path
@@ -332,7 +331,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["run", "pick", "events"])
     ap.add_argument("paths", nargs="*", help="run: MANIFEST; pick: TRACE MANIFEST")
-    ap.add_argument("--shapes", default="all", choices=["builtin", "pf", "all"])
+    ap.add_argument("--shapes", default="all", choices=["builtin", "gen", "all"])
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--write", action="store_true")
     ap.add_argument("--reps", type=int, default=5)
@@ -363,16 +362,13 @@ def main():
     single = pick_bins(measured)
     for (b, _, best) in single:
         print("bin mn 2^%d k 2^%d: tile %d parts %d" % (b[0], b[1], best[0], best[1]))
-    own = pick_shapes(measured, single)
-    for shape, best in own:
-        print("shape %s %s %d x %d x %d: tile %d parts %d" % (shape + best))
     if args.keep_grouped:
         grouped_entries = read_grouped_entries()
     else:
         grouped = GROUPED_SHAPES[::3] if args.quick else GROUPED_SHAPES
         grouped_entries = tune_grouped(h, torch, grouped, args.reps)
     if args.write:
-        print("wrote", write_table(single + grouped_entries, args.out, own))
+        print("wrote", write_table(single + grouped_entries, args.out))
     h.close()
 
 

@@ -186,7 +186,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         // The tuned table holds FP32-product measurements; other formats use the occupancy rule.
         // (An entry naming the one-launch kernel, tile 32 or 16, is used further down.)
         int tt = 0, tp = 0;
-        if (pl.format == FMT_F32 && tuned_lookup((int)p.ta, (int)p.tb, m, n, k, false, &tt, &tp) && tt >= 64)
+        if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp) && tt >= 64)
             tile = tt;
         else
             tile = tiles_for(128) >= cus ? 128 : 64;
@@ -204,7 +204,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         }
         pl.parts = std::min(cfg->kSplit, pl.kSteps);
     } else if (int tt = 0, tp = 0; pl.format == FMT_F32 && (!cfg || cfg->tile == 0) &&
-                                   tuned_lookup((int)p.ta, (int)p.tb, m, n, k, false, &tt, &tp) && tt >= 64) {
+                                   tuned_lookup(m, n, k, false, &tt, &tp) && tt >= 64) {
         pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
         // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition. FP32:
@@ -280,7 +280,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     // slice).
     {
         int tt = 0, tp = 0;
-        const bool listed = tuned_lookup((int)p.ta, (int)p.tb, m, n, k, false, &tt, &tp);
+        const bool listed = tuned_lookup(m, n, k, false, &tt, &tp);
         const long long tiles16 = ((m + 15) / 16) * ((n + 15) / 16);
         const bool fits = listed ? (tt == 32 || tt == 16) : tiles16 <= 256;
         const bool autoSmall = pl.format == FMT_F32 && !pinned && pl.identity != ID_FOUR && fits;

@@ -22,31 +22,9 @@
 
 namespace c3m {
 
-bool tuned_lookup(int opA, int opB, long long m, long long n, long long k, bool grouped, int* tile, int* parts)
+bool tuned_lookup(long long m, long long n, long long k, bool grouped, int* tile, int* parts)
 {
     if (m <= 0 || n <= 0 || k <= 0) return false;
-    if (!grouped) {
-        // A shape the tuner measured itself, where its plan differs from its class's: the same
-        // ops first; else, for a packed-tile plan, the same sizes under other ops (the pack
-        // turns every op into the same planes, so the packed plans rank alike whatever the ops;
-        // the one-launch kernel reads the operands as stored, so its entries need their ops).
-        const TunedShape* any = nullptr;
-        for (const TunedShape& e : kTunedShapes) {
-            if (e.tile == 0) break;
-            if (e.m != m || e.n != n || e.k != k) continue;
-            if (e.opA == opA && e.opB == opB) {
-                *tile = e.tile;
-                *parts = e.parts;
-                return true;
-            }
-            if (any == nullptr && e.tile >= 64) any = &e;
-        }
-        if (any != nullptr) {
-            *tile = any->tile;
-            *parts = any->parts;
-            return true;
-        }
-    }
     int mnl = 0, kl = 0;
     for (unsigned long long v = (unsigned long long)(m * n); v > 1; v >>= 1) ++mnl;
     for (unsigned long long v = (unsigned long long)k; v > 1; v >>= 1) ++kl;
@@ -227,7 +205,7 @@ aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCge
     } else if (over != nullptr && over->tile != 0) {
         cfg.tile = over->tile;
         cfg.kSplit = over->kSplit != 0 ? over->kSplit : 1;
-    } else if (tuned_lookup(0, 0, m, n, k, true, &tt, &tp)) {
+    } else if (tuned_lookup(m, n, k, true, &tt, &tp)) {
         small = tt == 32 || tt == 16;
         cfg.tile = small ? 0 : tt;
         cfg.kSplit = small ? 1 : tp;

@@ -150,9 +150,9 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& plan, c
 aclblasStatus_t validate_pointers(int k, const void* A, const void* B, const void* C, const char* op);
 bool is_zero(const aclblasComplex& z);
 
-// Autotuned plan table (tuned_table.h): tile and K partitions for a measured one-problem shape
-// (exact ops and sizes), else for its (log2 m*n, log2 k) bin.
-bool tuned_lookup(int opA, int opB, long long m, long long n, long long k, bool grouped, int* tile, int* parts);
+// Autotuned plan table (tuned_table.h): tile and K partitions for the (log2 m*n, log2 k) class
+// of a one-problem call or of a grouped member.
+bool tuned_lookup(long long m, long long n, long long k, bool grouped, int* tile, int* parts);
 
 // Temporary storage a call may use instead of the handle's workspace: a fixed device array
 // of the current device (base, capacity in bytes). Used by the no-allocation entry.

@@ -1,7 +1,10 @@
-"""The reference's performance suite on one MI355X: the 200 TC_PF cases (alpha = 1, beta = 0,
-shapes/ops/leading dimensions and the reference's GPU baseline times from
-tests/golden/pf_cases.csv, which holds the reference's ops-blas/test/gemm/cgemm3m/arch35/
-gpu_baseline.csv).
+#!/usr/bin/env python3
+"""The reference's performance suite on one MI355X, as a held-out check of the planner: the 200
+TC_PF cases (alpha = 1, beta = 0) with their shapes, ops, leading dimensions and the reference's
+GPU baseline times, read from the reference's own data file
+ops-blas/test/gemm/cgemm3m/arch35/gpu_baseline.csv, whose path is given on the command line
+(the file is not part of this repository; the planner's table is tuned on the tuner's own
+generated shapes, not on these).
 
 The reference's method (ops-blas/test/gemm/cgemm3m/README.md:95-115, cgemm3m_profile.h:32-71):
 5 warm-up and 10 sampled aclblasCgemm3m calls per case, each followed by a stream
@@ -10,8 +13,8 @@ profiler's kernel trace (launch gaps excluded); a case's time is the mean over t
 samples; the reported ratio is GPU_time / device_time (the reference's gate on its own
 device: every case >= 0.4, mean >= 0.5). Here the same two steps:
 
-  rocprofv3 --kernel-trace --output-format csv -d OUT -o run -- python3 -m cgemm3m.pf_suite run
-  python3 -m cgemm3m.pf_suite summary OUT/run_kernel_trace.csv profiles/pf_suite_vs_gpu_baseline.md
+  rocprofv3 --kernel-trace --output-format csv -d OUT -o run -- python3 pf_suite.py run CASES.csv
+  python3 pf_suite.py summary OUT/run_kernel_trace.csv CASES.csv profiles/pf_suite_vs_gpu_baseline.md
 
 `run` separates cases with one non-library kernel (a torch fill), so `summary` can split the
 trace: per case, the library's kernels form 15 calls of equal kernel count.
@@ -21,24 +24,26 @@ import math
 import os
 import sys
 
-HERE = os.path.dirname(os.path.abspath(__file__))
-ROOT = os.path.normpath(os.path.join(HERE, "..", ".."))
-CASES = os.path.join(ROOT, "tests", "golden", "pf_cases.csv")
+ROOT = os.path.dirname(os.path.abspath(__file__))
 WARMUP, SAMPLES = 5, 10
 
 
-def load_cases(path=CASES):
+def load_cases(path):
+    """Cases of the reference's gpu_baseline.csv (columns id, transa, transb, m, n, k, lda, ldb,
+    ldc, gpu_ms; ops written as ACLBLAS_OP_N / _T / _C)."""
     with open(path) as f:
         rows = list(csv.DictReader(f))
     for r in rows:
         for key in ("m", "n", "k", "lda", "ldb", "ldc"):
             r[key] = int(r[key])
+        for key in ("transa", "transb"):
+            r[key] = r[key].strip()[-1]
         r["gpu_ms"] = float(r["gpu_ms"])
     return rows
 
 
-def run(path=CASES):
-    sys.path.insert(0, os.path.dirname(HERE))
+def run(path):
+    sys.path.insert(0, os.path.join(ROOT, "ascend-open-source-contributions_amd"))
     import torch
 
     import cgemm3m
@@ -71,7 +76,7 @@ def run(path=CASES):
     print("pf suite: done", flush=True)
 
 
-def summary(trace_csv, out_md, path=CASES):
+def summary(trace_csv, path, out_md):
     cases = load_cases(path)
     groups, cur = [], None
     with open(trace_csv) as f:
@@ -105,17 +110,20 @@ def summary(trace_csv, out_md, path=CASES):
         lines.append((c, ours, ratio, per, kernels))
     gmean = math.exp(sum(math.log(x) for x in ratios) / len(ratios))
     trace_dir = os.path.dirname(os.path.abspath(trace_csv))
-    out = ["# The reference's 200 performance cases on one MI355X", "",
+    out = ["# The reference's 200 performance cases on one MI355X (held out from tuning)", "",
            "Cases, GPU baseline times and method from the reference (`ops-blas/test/gemm/cgemm3m/arch35/"
            "gpu_baseline.csv`, `ops-blas/test/gemm/cgemm3m/README.md:95-115`): alpha = 1, beta = 0; per case 5 "
            "warm-up + 10 sampled `aclblasCgemm3m` calls, each followed by a stream synchronisation; a call's time is "
            "the sum of its kernels' device durations (rocprofv3 kernel trace); the case time is the mean of the 10 "
            "samples; ratio = GPU baseline time / MI355X time (the reference's own device gate is ratio >= 0.4 per "
            "case, mean >= 0.5).", "",
-           "Commands (MI355X, one GPU):", "", "```",
+           "Held out: the planner's table (`csrc/tuned_table.h`) holds shape classes only, tuned on the tuner's "
+           "own seeded shape set (`autotune.generated_shapes`), none of these cases.", "",
+           "Commands (MI355X, one GPU; CASES = the reference's `ops-blas/test/gemm/cgemm3m/arch35/gpu_baseline.csv`):",
+           "", "```",
            "cd /tmp && TMPDIR=/tmp rocprofv3 --kernel-trace --output-format csv -d %s -o run -- "
-           "python3 -m cgemm3m.pf_suite run   # from ascend-open-source-contributions_amd/" % trace_dir,
-           "python3 -m cgemm3m.pf_suite summary %s %s" % (os.path.abspath(trace_csv), os.path.abspath(out_md)),
+           "python3 /root/repo/pf_suite.py run CASES" % trace_dir,
+           "python3 /root/repo/pf_suite.py summary %s CASES %s" % (os.path.abspath(trace_csv), os.path.abspath(out_md)),
            "```", "",
            "| | value |", "|---|---|",
            "| cases | %d |" % len(ratios),
@@ -139,12 +147,13 @@ def summary(trace_csv, out_md, path=CASES):
 
 
 def main():
-    if len(sys.argv) >= 2 and sys.argv[1] == "run":
-        run(sys.argv[2] if len(sys.argv) > 2 else CASES)
-    elif len(sys.argv) >= 4 and sys.argv[1] == "summary":
-        summary(sys.argv[2], sys.argv[3])
+    if len(sys.argv) == 3 and sys.argv[1] == "run":
+        run(sys.argv[2])
+    elif len(sys.argv) == 5 and sys.argv[1] == "summary":
+        summary(sys.argv[2], sys.argv[3], sys.argv[4])
     else:
-        raise SystemExit("usage: python -m cgemm3m.pf_suite run [cases.csv] | summary TRACE.csv OUT.md")
+        raise SystemExit("usage: python pf_suite.py run CASES.csv | summary TRACE.csv CASES.csv OUT.md\n"
+                         "(CASES.csv: the reference's ops-blas/test/gemm/cgemm3m/arch35/gpu_baseline.csv)")
 
 
 if __name__ == "__main__":

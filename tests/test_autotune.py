@@ -1,6 +1,7 @@
-"""The autotuner's host logic (no GPU): plan choice with its noise margin, per-class and
-per-shape picks, the table it writes, and the kernel-trace reader."""
+"""The autotuner's host logic (no GPU): plan choice with its noise margin, per-class picks,
+its own generated shape set, the table it writes, and the kernel-trace reader."""
 import csv
+import math
 import re
 
 from cgemm3m import autotune as at
@@ -28,38 +29,37 @@ def test_pick_bins_minimises_geometric_mean_of_time_over_best():
         (("N", "N", 512, 512, 200), {(32, 1): 1.0, (16, 1): 1.5}),
         (("T", "N", 500, 600, 150), {(32, 1): 1.1, (16, 1): 1.0}),
     ]
-    single = at.pick_bins(measured)
-    assert single == [((18, 7), 0, (32, 1))]
-    own = at.pick_shapes(measured, single)
-    assert own == [(("T", "N", 500, 600, 150), (16, 1))]
+    assert at.pick_bins(measured) == [((18, 7), 0, (32, 1))]
 
 
-def test_pick_shapes_keeps_first_measurement_of_a_repeated_shape():
-    measured = [
-        (("N", "N", 64, 64, 64), {(32, 1): 1.0, (16, 1): 2.0}),
-        (("N", "N", 64, 64, 64), {(32, 1): 2.0, (16, 1): 1.0}),
-    ]
-    single = [((12, 6), 0, (16, 1))]
-    assert at.pick_shapes(measured, single) == [(("N", "N", 64, 64, 64), (32, 1))]
+def test_generated_shapes_are_seeded_and_cover_the_range():
+    a, b = at.generated_shapes(), at.generated_shapes()
+    assert a == b and len(a) == 320
+    assert {s[0] + s[1] for s in a} == {x + y for x in "NTC" for y in "NTC"}
+    dims = [d for s in a for d in s[2:5]]
+    assert min(dims) == 1 or min(dims) <= 2
+    assert max(dims) <= 4096 and max(dims) >= 3500
+    for ta, tb, m, n, k, lda, ldb, ldc in a:
+        assert lda >= (m if ta == "N" else k) and ldb >= (k if tb == "N" else n) and ldc >= m
+    # at least a few shapes in each decade of m*n*k, from tiny to 4096^3-sized
+    mnk = [math.log10(s[2] * s[3] * s[4]) for s in a]
+    assert sum(x < 3 for x in mnk) >= 3 and sum(x > 10 for x in mnk) >= 3
 
 
 def test_write_table_round_trip(tmp_path):
     entries = [((18, 7), 0, (32, 1)), ((12, 6), 1, (16, 1)), ((20, 9), 1, (128, 2))]
-    shapes = [(("T", "N", 581, 782, 175), (16, 1)), (("N", "C", 100, 70, 40), (64, 2))]
     path = tmp_path / "table.h"
-    at.write_table(entries, str(path), shapes)
+    at.write_table(entries, str(path))
     text = path.read_text()
     assert "{18, 7, 0, 32, 1}," in text and "{12, 6, 1, 16, 1}," in text
-    # shapes sorted by (m, n, k), ops as the C enum values
-    i, j = text.index("{100, 70, 40, 111, 113, 64, 2},"), text.index("{581, 782, 175, 112, 111, 16, 1},")
-    assert i < j
+    assert "kTunedShapes" not in text
     assert sorted(at.read_grouped_entries(str(path))) == [((12, 6), 1, (16, 1)), ((20, 9), 1, (128, 2))]
 
 
 def test_installed_table_is_consistent():
     text = open(at.TABLE).read()
-    classes = text[text.index("kTunedTable[]"):text.index("kTunedShapes[]")]
-    shapes = text[text.index("kTunedShapes[]"):]
+    assert "kTunedShapes" not in text  # classes only: no plan for any single shape
+    classes = text[text.index("kTunedTable[]"):]
     bins = {}
     for a, b, g, t, p in re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", classes):
         if int(t):
@@ -67,16 +67,6 @@ def test_installed_table_is_consistent():
             assert key not in bins, key
             assert int(t) in (16, 32, 64, 128) and 1 <= int(p) <= 16
             bins[key] = (int(t), int(p))
-    seen = set()
-    for m, n, k, oa, ob, t, p in re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)\}", shapes):
-        m, n, k, oa, ob, t, p = (int(x) for x in (m, n, k, oa, ob, t, p))
-        if t == 0:
-            continue
-        assert oa in (111, 112, 113) and ob in (111, 112, 113)
-        assert (m, n, k, oa, ob) not in seen
-        seen.add((m, n, k, oa, ob))
-        # an override exists only where the shape's own plan differs from its class's
-        assert bins.get(at.bin_of(m, n, k) + (0,)) != (t, p), (m, n, k)
 
 
 def test_read_trace_groups_candidates_and_skips_the_pool_clear(tmp_path):

@@ -229,19 +229,16 @@ def test_cgemm_status_contract(h, kw, expect):
     assert cg_call(h, **kw) == expect
 
 
-def _installed_tables():
+def _installed_classes():
     from cgemm3m import autotune
 
     text = open(autotune.TABLE).read()
-    classes = text[text.index("kTunedTable[]"):text.index("kTunedShapes[]")]
-    shapes = text[text.index("kTunedShapes[]"):]
+    classes = text[text.index("kTunedTable[]"):]
     cls = {}
     for a, b, g, t, p in re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", classes):
         if int(t):
             cls[(int(a), int(b), int(g))] = (int(t), int(p))
-    shp = [tuple(int(x) for x in e) for e in
-           re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+), (\d+), (\d+)\}", shapes) if int(e[5])]
-    return cls, shp
+    return cls
 
 
 def _planned(h, ta, tb, m, n, k):
@@ -262,33 +259,23 @@ def _expect(plan, tile, parts, k):
         assert plan["path"] == 5 and plan["parts"] == min(parts, (k + 15) // 16), plan
 
 
-def test_planner_follows_tuned_shapes_and_classes(h):
-    """The planner's table lookup against the installed table: every tuned shape under its
-    own ops gets its entry's plan; a packed-tile entry serves its sizes under other ops (when
-    those have no entry of their own); a shape without an entry gets its class's plan."""
+def test_planner_follows_tuned_classes(h):
+    """The planner's table lookup against the installed table: a shape of every one-problem
+    class gets its class's plan, under every op pair (the table holds classes only)."""
     import torch
 
     if torch.cuda.is_available():
         pytest.skip("needs a host without a GPU (the call must stop after planning)")
     from cgemm3m import autotune
 
-    names = {111: "N", 112: "T", 113: "C"}
-    cls, shp = _installed_tables()
-    exact = {(m, n, k, oa, ob) for (m, n, k, oa, ob, _, _) in shp}
-    for m, n, k, oa, ob, t, p in shp:
-        _expect(_planned(h, names[oa], names[ob], m, n, k), t, p, k)
-        if t >= 64 and (m, n, k, 113, 113) not in exact:
-            first = next(e for e in shp if e[:3] == (m, n, k) and e[5] >= 64)
-            _expect(_planned(h, "C", "C", m, n, k), first[5], first[6], k)
     checked = 0
-    for (mnl, kl, g), (t, p) in sorted(cls.items()):
+    for (mnl, kl, g), (t, p) in sorted(_installed_classes().items()):
         if g:
             continue
         m, n, k = 2 ** (mnl // 2), 2 ** (mnl - mnl // 2), 2 ** kl  # a shape of that class
-        if any(e[:3] == (m, n, k) for e in shp):
-            continue
         assert autotune.bin_of(m, n, k) == (mnl, kl)
-        _expect(_planned(h, "N", "T", m, n, k), t, p, k)
+        for ops in ("NT", "CN", "TC"):
+            _expect(_planned(h, ops[0], ops[1], m, n, k), t, p, k)
         checked += 1
     assert checked >= 20, checked
 
