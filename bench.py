@@ -383,10 +383,14 @@ def cpu_baseline(A, B, C0, m, n, k, alpha, beta, seconds):
         if elapsed * 4 > seconds or cols >= n:
             break
         cols = min(n, max(cols * 2, int(cols * seconds / max(elapsed, 1e-3) * 0.9)))
-    return {"value": round(rate / 1e12, 6), "unit": "complex TFLOP/s (8mnk)", "cores": threads, "kind": "port",
+    # "cores" is the threads used (the contract's meaning); the host's own count beside it
+    return {"value": round(rate / 1e12, 6), "unit": "complex TFLOP/s (8mnk)", "cores": threads, "threads": threads,
+            "host_cores": os.cpu_count(), "kind": "port",
             "sample": "Netlib-order cgemm restatement (oracle/oracle.c, -O2 -ffp-contract=off, OpenMP over "
-                      "columns) on the first %d of %d output columns of the same %d^3 NN inputs, %.1f s"
-                      % (cols, n, m, elapsed)}
+                      "columns, %d threads on a host with %d cores) on the first %d of %d output columns of the "
+                      "same %d^3 NN inputs, %.1f s; the rate is 8*m*cols*k over that time (Netlib's columns are "
+                      "independent, so the rate of a column sample is the whole product's)"
+                      % (threads, os.cpu_count(), cols, n, m, elapsed)}
 
 
 if __name__ == "__main__":
