@@ -17,6 +17,7 @@
 // path: products combined, alpha/beta applied and C written in registers, with the
 // rounding-correction screen. PARTIAL=true is the K-split path: partition p of the k range
 // writes its (re, im) partial planes; merge_kernel sums partitions in index order.
+#include <type_traits>
 #include "device_util.h"
 #include "launch.h"
 
@@ -316,6 +317,226 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm3m_f32_direct_kernel(Ge
     f32_tile_direct<BM, BN, PARTIAL, WM, WN, FOUR>(g, part, tmI, tnI);
 }
 
+// Raw-operand form of the 64 x 64 wave tile, for grouped members: the wave reads its operands
+// straight from the caller's interleaved complex64 A and B and forms the plane values in
+// registers, so the batch needs no pack launch and no plane workspace. Per 16-deep stage a
+// lane holds, for each of its two 32-row fragments of op(A) (and two 32-column fragments of
+// op(B)) and each 8-deep k fragment, the four consecutive k = 4 * (lane / 32) + s that the
+// packed fragment's slots hold: four 8-byte loads (32 lanes cover 32 consecutive rows when
+// rows are adjacent in memory, one row's 32 contiguous bytes per lane otherwise).
+// Out-of-range rows and k (up to the packed extent kPad, the same stage count) read a clamped
+// in-range address and become zeros; conjugation negates the imaginary part; the plane values
+// use the pack kernel's formulas (plane_value). So every MFMA operand and the MFMA order per
+// accumulator equal the packed 64 x 64 tile's: the results are byte-identical to it.
+typedef float rawf2 __attribute__((ext_vector_type(2)));  // one complex64 element (re, im)
+
+template <int ID>
+struct RawSet {
+    rawf2 a[2][2][4], b[2][2][4];  // [k fragment][32-row / 32-column fragment][k slot]
+};
+
+template <bool PARTIAL, int ID>
+__device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int tmI, int tnI)
+{
+    constexpr int TM = 2, TN = 2, KFD = 2, NSET = 2;
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int kf0 = 2 * (int)((long long)g.kSteps * part / g.parts);
+    const int kf1 = 2 * (int)((long long)g.kSteps * (part + 1) / g.parts);
+    const int K = g.src.k;
+    // op(A)(i, k) = A[i + k*lda] (N) or A[k + i*lda] (T, C); op(B)(k, j) = B[k + j*ldb] (N) or
+    // B[j + k*ldb] (T, C). Byte address = tile base (the tile's first row or column) + k * step
+    // (uniform per stage and slot) + a 32-bit lane offset fixed for the tile (row or column
+    // within the tile, and the lane's k offset); the host sends a batch here only if every such
+    // offset fits in 31 bits.
+    // Rows (columns) past the matrix read the last one: they feed only outputs that are never
+    // stored, and the fused store writes in-range outputs only (a K-split partial plane holds
+    // them in its padding, which the merge does not read).
+    const bool aKc = g.src.opA != 111, bKc = g.src.opB == 111;
+    const long long aStepB = aKc ? 8 : g.src.lda * 8, bStepB = bKc ? 8 : g.src.ldb * 8;
+    const int i0 = tmI * 64, j0 = tnI * 64;
+    const char* aTile = reinterpret_cast<const char*>(g.src.A) + (aKc ? (long long)i0 * g.src.lda * 8 : (long long)i0 * 8);
+    const char* bTile = reinterpret_cast<const char*>(g.src.B) + (bKc ? (long long)j0 * g.src.ldb * 8 : (long long)j0 * 8);
+    const float aSign = g.src.opA == 113 ? -1.0f : 1.0f, bSign = g.src.opB == 113 ? -1.0f : 1.0f;
+    unsigned aOff[TM], bOff[TN];
+#pragma unroll
+    for (int b = 0; b < TM; ++b) {
+        const long long d = min(i0 + 32 * b + (lane & 31), g.m - 1) - i0;
+        aOff[b] = (unsigned)((aKc ? d * g.src.lda * 8 : d * 8) + 4 * half * aStepB);
+    }
+#pragma unroll
+    for (int a = 0; a < TN; ++a) {
+        const long long d = min(j0 + 32 * a + (lane & 31), g.n - 1) - j0;
+        bOff[a] = (unsigned)((bKc ? d * g.src.ldb * 8 : d * 8) + 4 * half * bStepB);
+    }
+    typedef __attribute__((address_space(1))) const rawf2 gf2;
+    typedef float rawf4 __attribute__((ext_vector_type(4), aligned(8)));
+    auto at = [](const char* base, unsigned off) { return *(gf2*)(base + off); };
+    // Buffer loads for the ring: resource = the stage's uniform base (64-bit scalar math once
+    // per stage and slot), 32-bit lane offset in a VGPR, small uniform offset in an SGPR, so a
+    // load costs no address arithmetic. No range check (num_records = 2^32 - 1): every offset
+    // is in bounds by construction (clamped rows, stages wholly below K).
+    auto rsrc = [](const char* base) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)0xffffffff, 0x00020000);
+    };
+    auto bl2 = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(rawf2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    };
+    auto bl4 = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(rawf4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    };
+    // One stage: 16 k from k = (kf0 + st * KFD) * 8, the lane's k at 4 * (lane / 32) + u within
+    // each 8. Loads only fill registers (no branch around or between loads, no use of a loaded
+    // value here): anything consuming a look-ahead load in the load block, or a branch joining
+    // loads, would make the wait for it come at once, so the operand layouts are compile-time
+    // (AKC, BKC: op(A), op(B) k-contiguous, whose lane's four k are 32 contiguous bytes, read
+    // as two 16-byte loads; the other layout reads one k per load, rows adjacent across lanes).
+    auto load = [&](RawSet<ID>& S, int st, auto aTag, auto bTag) {
+        constexpr bool AKC = decltype(aTag)::value, BKC = decltype(bTag)::value;
+        const int sk = (kf0 + st * KFD) * 8;
+#pragma unroll
+        for (int kfl = 0; kfl < KFD; ++kfl) {
+            const __amdgpu_buffer_rsrc_t ar = rsrc(aTile + (long long)(sk + kfl * 8) * aStepB);
+            const __amdgpu_buffer_rsrc_t br = rsrc(bTile + (long long)(sk + kfl * 8) * bStepB);
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int u = 0; u < 4; u += (AKC ? 2 : 1)) {
+                    if constexpr (AKC) {
+                        const rawf4 w = bl4(ar, aOff[b], 8 * u);
+                        S.a[kfl][b][u] = rawf2{w[0], w[1]};
+                        S.a[kfl][b][u + 1] = rawf2{w[2], w[3]};
+                    } else {
+                        S.a[kfl][b][u] = bl2(ar, aOff[b], (unsigned)(u * aStepB));
+                    }
+                }
+#pragma unroll
+            for (int a = 0; a < TN; ++a)
+#pragma unroll
+                for (int u = 0; u < 4; u += (BKC ? 2 : 1)) {
+                    if constexpr (BKC) {
+                        const rawf4 w = bl4(br, bOff[a], 8 * u);
+                        S.b[kfl][a][u] = rawf2{w[0], w[1]};
+                        S.b[kfl][a][u + 1] = rawf2{w[2], w[3]};
+                    } else {
+                        S.b[kfl][a][u] = bl2(br, bOff[a], (unsigned)(u * bStepB));
+                    }
+                }
+        }
+    };
+    // The stage reaching past K: k clamped to K - 1 per element (the lane's k offset sits in the
+    // lane offsets; a clamped k, per lane, takes it out of them again), one k per load.
+    auto loadClamped = [&](RawSet<ID>& S, int st) {
+        const int sk = (kf0 + st * KFD) * 8;
+#pragma unroll
+        for (int kfl = 0; kfl < KFD; ++kfl)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long kr = min(sk + kfl * 8 + 4 * half + u, K - 1) - 4 * half;
+                const char* ar = aTile + kr * aStepB;
+                const char* br = bTile + kr * bStepB;
+#pragma unroll
+                for (int b = 0; b < TM; ++b) S.a[kfl][b][u] = at(ar, aOff[b]);
+#pragma unroll
+                for (int a = 0; a < TN; ++a) S.b[kfl][a][u] = at(br, bOff[a]);
+            }
+    };
+
+    f32x16 acc[3][TN][TM];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[q][a][b][r] = 0.0f;
+
+    // Plane values of one element with the conjugation sign folded in: s * im is exact (s = +-1),
+    // so fma(s, im, x) rounds exactly as x + (s * im) does, and the values equal
+    // plane_value(role, ID, q, re, s * im), the pack kernel's.
+    auto planeA = [&](int q, float re, float im) {
+        if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? aSign * im : fmaf(aSign, im, re));
+        return q == 0 ? fmaf(aSign, im, re) : (q == 1 ? re : aSign * im);  // (Ar+Ai, Ar, Ai)
+    };
+    auto planeB = [&](int q, float re, float im) {
+        if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? bSign * im : fmaf(bSign, im, re));
+        return q == 0 ? re : (q == 1 ? fmaf(bSign, im, -re) : fmaf(bSign, im, re));  // (Br, Bi-Br, Br+Bi)
+    };
+    // MASK: k past K contributes zeros (both operands, so an Inf or NaN read from the clamped
+    // address adds nothing).
+    auto mfma = [&](const RawSet<ID>& S, int st, auto maskTag) {
+        constexpr bool MASK = decltype(maskTag)::value;
+        const int sk = (kf0 + st * KFD) * 8;
+#pragma unroll
+        for (int kfl = 0; kfl < KFD; ++kfl)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const bool kin = !MASK || sk + kfl * 8 + 4 * half + s < K;
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int a = 0; a < TN; ++a)
+#pragma unroll
+                        for (int b = 0; b < TM; ++b) {
+                            const rawf2 av = S.a[kfl][b][s], bv = S.b[kfl][a][s];
+                            const float x = kin ? planeB(q, bv.x, bv.y) : 0.0f;
+                            const float y = kin ? planeA(q, av.x, av.y) : 0.0f;
+                            acc[q][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, acc[q][a][b], 0, 0, 0);
+                        }
+            }
+    };
+    typedef std::integral_constant<bool, false> Off;
+    typedef std::integral_constant<bool, true> On;
+
+    // Stages wholly below K run through the ring: NSET register sets, unconditional look-ahead
+    // loads (the last full stage again past the end), scheduling barriers keeping each stage's
+    // loads ahead of its MFMAs. The stage (at most the partition's last) reaching past K
+    // follows, clamped and masked.
+    const int nst = (kf1 - kf0) / KFD;
+    const int nFull = min(nst, max(0, (K - kf0 * 8) / (8 * KFD)));
+    auto ring = [&](auto aTag, auto bTag) {
+        RawSet<ID> S[NSET];
+        if (nFull > 0) {
+#pragma unroll
+            for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nFull - 1), aTag, bTag);
+        }
+        int st = 0;
+        for (; st + NSET <= nFull; st += NSET) {
+#pragma unroll
+            for (int u = 0; u < NSET; ++u) {
+                load(S[(u + NSET - 1) % NSET], min(st + u + NSET - 1, nFull - 1), aTag, bTag);
+                __builtin_amdgcn_sched_barrier(0);
+                mfma(S[u], st + u, Off());
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u + 1 < NSET; ++u)
+            if (st + u < nFull) mfma(S[u], st + u, Off());
+    };
+    if (aKc) {
+        if (bKc)
+            ring(On(), On());
+        else
+            ring(On(), Off());
+    } else {
+        if (bKc)
+            ring(Off(), On());
+        else
+            ring(Off(), Off());
+    }
+    for (int t = nFull; t < nst; ++t) {
+        RawSet<ID> T;
+        loadClamped(T, t);
+        mfma(T, t, On());
+    }
+
+    const int iBase = i0 + (lane & 31);
+    const int jBase = j0 + 4 * half;
+    store_tile<TN, TM, PARTIAL, 3>(g, acc, part, iBase, jBase);
+}
+
 // Grouped batch on 64 x 64 wave tiles: a persistent grid of four-wave workgroups pulls quads
 // (four wave tiles, entries 4q .. 4q + 3 of the list) from one queue; wave w computes entry
 // 4q + w with the per-wave code of the 128 x 128 LDS-free kernel (a 64 x 64 output block,
@@ -324,8 +545,9 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm3m_f32_direct_kernel(Ge
 // share operand fragments in the CU's cache as they do there; the blocks at the edges of the
 // members (where a 128 x 128 tile would be mostly padding) are grouped four at a time by
 // cost. An entry with problem < 0 is an empty slot. Quads are sorted by decreasing cost.
-__global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_quad_kernel(const GroupProblem* probs,
-                                                                        const GroupTile* tiles, int nquads,
+template <bool RAW>
+__global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_quad_kernel(const GroupProblem* __restrict__ probs,
+                                                                        const GroupTile* __restrict__ tiles, int nquads,
                                                                         unsigned int* queue)
 {
     __shared__ int next;
@@ -343,17 +565,36 @@ __global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_quad_kernel(const G
         const int part = __builtin_amdgcn_readfirstlane(gt.part);
         const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
         const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
-        if (g.parts > 1)
+        if constexpr (RAW) {
+            const bool std3m = __builtin_amdgcn_readfirstlane(g.identity) == ID_STANDARD;
+            if (g.parts > 1) {
+                if (std3m)
+                    f32_tile_raw64<true, ID_STANDARD>(g, part, tm, tn);
+                else
+                    f32_tile_raw64<true, ID_FUSED>(g, part, tm, tn);
+            } else {
+                if (std3m)
+                    f32_tile_raw64<false, ID_STANDARD>(g, part, tm, tn);
+                else
+                    f32_tile_raw64<false, ID_FUSED>(g, part, tm, tn);
+            }
+        } else if (g.parts > 1) {
             f32_tile_direct<64, 64, true, 1, 1, false, C3M_DIRECT_KFD, C3M_DIRECT_NSET, true>(g, part, tm, tn);
-        else
+        } else {
             f32_tile_direct<64, 64, false, 1, 1, false, C3M_DIRECT_KFD, C3M_DIRECT_NSET, true>(g, part, tm, tn);
+        }
     }
 }
 
 hipError_t launch_gemm_f32_grouped_quads(const GroupProblem* probs, const GroupTile* tiles, int nquads,
-                                         unsigned int* queue, int blocks, hipStream_t s)
+                                         unsigned int* queue, int blocks, bool raw, hipStream_t s)
 {
-    hipLaunchKernelGGL(gemm3m_f32_grouped_quad_kernel, dim3(blocks), dim3(256), 0, s, probs, tiles, nquads, queue);
+    if (raw)
+        hipLaunchKernelGGL(gemm3m_f32_grouped_quad_kernel<true>, dim3(blocks), dim3(256), 0, s, probs, tiles, nquads,
+                           queue);
+    else
+        hipLaunchKernelGGL(gemm3m_f32_grouped_quad_kernel<false>, dim3(blocks), dim3(256), 0, s, probs, tiles, nquads,
+                           queue);
     return hipGetLastError();
 }
 

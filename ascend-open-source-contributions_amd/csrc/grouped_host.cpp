@@ -181,11 +181,38 @@ struct GroupedLayout {
     size_t descBytes = 0, offQueue = 0, total = 0;
     size_t nProb = 0, nPack = 0, nQuad = 0, nMerge = 0;
     size_t nCorr = 0;               // packed members with the rounding correction on
+    size_t energyBytes = 0;         // descriptor bytes of their energy items
     long long packBlocks = 0;
     int maxMergeOutputs = 0;
+    bool raw = true;                // the quad kernel reads the members' operands as stored
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Workspace of a member the quad kernel reads raw: no planes, only the K-split partial planes
+// and, with the correction on, the energy partials (from an energy-only pass of the pack
+// kernel, so the energies equal the packed form's bit for bit), the energies and the bitmap.
+static void raw_member_workspace(Plan& pl)
+{
+    size_t total = 0;
+    auto region = [&](size_t bytes) {
+        const size_t off = align256(total);
+        total = off + bytes;
+        return off;
+    };
+    pl.offA = pl.offB = pl.offEPA = pl.offEPB = 0;
+    pl.aPlaneBytes = pl.bPlaneBytes = 0;  // no planes: pack items of this member write none
+    if (pl.parts > 1) pl.offPartial = region(pl.partStride * 4 * (size_t)pl.parts);
+    if (pl.correction) {
+        pl.offEPA = region((size_t)pl.kTiles * pl.mPad * 8);
+        pl.offEPB = region((size_t)pl.kTiles * pl.nPad * 8);
+        pl.offEA = region((size_t)pl.mPad * 8);
+        pl.offEB = region((size_t)pl.nPad * 8);
+        pl.offBitmap = region((size_t)pl.bitmapWords * 8);
+        pl.offCounter = region(256);
+    }
+    pl.total = total;
+}
 
 // Plan of one grouped member: the tuned table's plan for grouped members (tile 64 / 128 with
 // K partitions, or the one-launch kernel, tile 32 / 16, unsliced), else 128 tiles for members
@@ -231,6 +258,12 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
                                     aclblasComplex* const* C, const int* ldc, bool withPointers,
                                     const aclblasCgemm3mConfig* over, GroupedLayout& L, const char* op)
 {
+    // variant 7 keeps the earlier form: members packed into planes by one launch first
+    L.raw = !(over != nullptr && over->variant == 7);
+    // The raw quad kernel addresses a tile's elements by 32-bit byte offsets (at most 68 leading
+    // dimensions of 8 bytes from the tile's base); wider leading dimensions take the packed form.
+    for (int i = 0; i < count && withPointers; ++i)
+        if ((long long)std::max(lda[i], ldb[i]) * 8 * 68 >= (1LL << 31)) L.raw = false;
     for (int i = 0; i < count; ++i) {
         const aclblasStatus_t st =
             validate(h, ta[i], tb[i], m[i], n[i], k[i], &alpha[i], withPointers ? lda[i] : std::max(1, ta[i] != ACLBLAS_OP_N ? k[i] : m[i]),
@@ -258,6 +291,7 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             L.smallPlans.push_back(pl);
             continue;
         }
+        if (L.raw) raw_member_workspace(pl);
         L.probs.push_back(p);
         L.plans.push_back(pl);
     }
@@ -279,10 +313,11 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
     }
     L.nQuad += (loose + 3) / 4;
     L.nProb = L.probs.size();
-    L.nPack = 2 * L.nProb;
+    L.nPack = 2 * (L.raw ? L.nCorr : L.nProb);  // raw: energy-only items of the corrected members
+    L.energyBytes = L.nCorr * sizeof(EnergyArgs);
     L.descBytes = align256(L.nProb * sizeof(GroupProblem)) + align256(L.nPack * sizeof(GroupPack)) +
                   align256(4 * L.nQuad * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge)) +
-                  align256(L.small.size() * sizeof(GroupSmall)) + align256(L.nCorr * sizeof(EnergyArgs)) +
+                  align256(L.small.size() * sizeof(GroupSmall)) + align256(L.energyBytes) +
                   align256(L.nCorr * sizeof(RecomputeArgs));
     L.offQueue = total;
     total += 256;
@@ -382,11 +417,11 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         std::vector<Cost> loose;
         quads.reserve(L.nQuad);
         // members with the rounding correction: energies (after the pack) and recompute (last)
-        std::vector<EnergyArgs> energies;
+        std::vector<EnergyArgs> energies;  // from the pack's (or the energy-only pass's) partials
         std::vector<RecomputeArgs> recomputes;
         long long maxEnergyWork = 0, maxWords = 0;
         long long packBlocks = 0;
-        size_t nm = 0;
+        size_t nm = 0, npk = 0;
         for (size_t i = 0; i < L.nProb; ++i) {
             const Problem& p = L.probs[i];
             const Plan& pl = L.plans[i];
@@ -410,21 +445,23 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 a.format = pl.format;
                 a.pieces = pl.pieces;
                 a.nplanes = 3;  // grouped members use a 3M identity
-                a.planes = pws + (isA ? pl.offA : pl.offB);
+                a.planes = L.raw ? nullptr : pws + (isA ? pl.offA : pl.offB);
                 a.planeBytes = isA ? pl.aPlaneBytes : pl.bPlaneBytes;
                 a.energyPartial =
                     pl.correction ? reinterpret_cast<double*>(pws + (isA ? pl.offEPA : pl.offEPB)) : nullptr;
                 a.scaleExp = nullptr;
-                pk[2 * i + role].a = a;
-                pk[2 * i + role].firstBlock = packBlocks;
+                if (L.raw && !pl.correction) continue;  // read as stored by the quad kernel
+                pk[npk].a = a;
+                pk[npk].firstBlock = packBlocks;
+                ++npk;
                 packBlocks += (long long)(a.mnPad / 64) * ((a.kPad + 63) / 64);
             }
             GemmArgs& g = gp[i].g;
             g = GemmArgs{};
-            g.Ap = pa.planes;
+            g.Ap = L.raw ? nullptr : pa.planes;
             g.aPlaneBytes = pl.aPlaneBytes;
             g.mPad = pl.mPad;
-            g.Bp = pb.planes;
+            g.Bp = L.raw ? nullptr : pb.planes;
             g.bPlaneBytes = pl.bPlaneBytes;
             g.nPad = pl.nPad;
             g.m = p.m;
@@ -581,13 +618,15 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 ++nSmall[form];
             }
         const size_t offEnergy = offSmall + align256(L.small.size() * sizeof(GroupSmall));
-        const size_t offRecompute = offEnergy + align256(L.nCorr * sizeof(EnergyArgs));
-        if (energies.size() != L.nCorr || recomputes.size() != L.nCorr) {
-            log_error(op, "internal error: %zu corrected members laid out, %zu counted", energies.size(), L.nCorr);
+        const size_t offRecompute = offEnergy + align256(L.energyBytes);
+        unsigned int* queue = L.nProb > 0 ? reinterpret_cast<unsigned int*>(ws + L.offQueue) : nullptr;
+        if (energies.size() != L.nCorr || recomputes.size() != L.nCorr || npk != L.nPack) {
+            log_error(op, "internal error: %zu corrected members and %zu pack items laid out, %zu and %zu counted",
+                      energies.size(), npk, L.nCorr, L.nPack);
             return ACLBLAS_STATUS_INTERNAL_ERROR;
         }
         if (L.nCorr > 0) {
-            std::memcpy(blob.data() + offEnergy, energies.data(), L.nCorr * sizeof(EnergyArgs));
+            std::memcpy(blob.data() + offEnergy, energies.data(), L.energyBytes);
             std::memcpy(blob.data() + offRecompute, recomputes.data(), L.nCorr * sizeof(RecomputeArgs));
         }
 
@@ -634,14 +673,18 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             }
         }
         if (fork && e == hipSuccess) e = hipEventRecord(handle->joinEv, ss);
-        unsigned int* queue = L.nProb > 0 ? reinterpret_cast<unsigned int*>(ws + L.offQueue) : nullptr;
         const GroupProblem* dProb = reinterpret_cast<const GroupProblem*>(dDesc);
         const GroupPack* dPack = reinterpret_cast<const GroupPack*>(dDesc + offPack);
         const GroupTile* dTiles = reinterpret_cast<const GroupTile*>(dDesc + offTiles);
         const GroupMerge* dMerge = reinterpret_cast<const GroupMerge*>(dDesc + offMerge);
-        if (e == hipSuccess && L.nProb > 0) {
+        if (e == hipSuccess && L.nPack > 0) {
             StageTimer t(handle, ST_PACK);
-            e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);
+            if (L.raw)  // energy partials of the corrected members only
+                e = launch_energy_partial_grouped(dPack, (int)L.nPack, packBlocks, queue, s);
+            else
+                e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);  // also clears the queue
+        } else if (e == hipSuccess && L.nProb > 0) {
+            e = launch_fill_u64(reinterpret_cast<unsigned long long*>(queue), 1, s);  // the tile queue
         }
         if (e == hipSuccess && L.nCorr > 0) {
             StageTimer t(handle, ST_ENERGY);
@@ -653,7 +696,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             const int cus = handle->cus > 0 ? handle->cus : 256;
             if (!quads.empty())
                 e = launch_gemm_f32_grouped_quads(dProb, dTiles, (int)quads.size(), queue,
-                                                  std::min<int>((int)quads.size(), cus), s);
+                                                  std::min<int>((int)quads.size(), cus), L.raw, s);
         }
         if (e == hipSuccess && nm > 0) {
             StageTimer t(handle, ST_MERGE);
@@ -712,10 +755,12 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle
 {
     const char* op = "aclblasCgemm3mGroupedBatchedEx";
     // variant 5 pins the one-launch kernel for every member (tile 16: its single-wave form;
-    // members are not k-sliced); otherwise tile 64 / 128 and kSplit pin the packed kernels
+    // members are not k-sliced); otherwise tile 64 / 128 and kSplit pin the quad kernel's
+    // plan; variant 7 packs the quad members into planes first instead of reading them raw
     const bool small = config != nullptr && config->variant == 5;
     if (config != nullptr &&
-        ((small ? (config->tile != 0 && config->tile != 16) || config->kSplit > 1
+        ((config->variant != 0 && config->variant != 5 && config->variant != 7) ||
+         (small ? (config->tile != 0 && config->tile != 16) || config->kSplit > 1
                 : config->tile != 0 && config->tile != 64 && config->tile != 128) ||
          config->kSplit < 0 || config->kSplit > 16 || config->identity < 0 || config->identity > ID_STANDARD ||
          config->correction < -1 || config->correction > 2)) {

@@ -46,6 +46,7 @@ struct PackArgs {
     int pieces;           // BF16: 2 (hi, lo) or 3 (hi, mid, lo); F32/F16: 1
     int nplanes;          // planes written: 3, or 2 for the B operand of ID_FOUR
     void* planes;         // base of plane 0; plane q piece p at planes + (q*pieces+p)*planeBytes
+                          // (nullptr: the energy partials only)
     size_t planeBytes;
     double* energyPartial; // optional [kPad/64][mnPad]: sum over 64 k of |x|^2 per row
     const int* scaleExp;   // FP16: per (plane, mn/128, k/128) exponent (from the scale pass)

@@ -64,6 +64,7 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
                 ((esum[tid] + esum[64 + tid]) + esum[128 + tid]) + esum[192 + tid];
     }
 
+    if (a.planes == nullptr) return;  // energy partials only (grouped members read raw)
     const size_t mnFrags = (size_t)(a.mnPad / 32);
     if (a.format == FMT_F32) {
         // 8 k fragments x 2 mn fragments x 64 lanes; lane holds 4 consecutive k.
@@ -159,7 +160,8 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-// Grouped batch: one launch packs both operands of every problem. Workgroup b finds its
+// Grouped batch: one launch packs both operands of every problem (or, for members the quad
+// kernel reads raw, only computes their energy partials). Workgroup b finds its
 // (problem, operand) item by binary search over the items' first-workgroup prefix; the
 // first workgroup also resets the grouped GEMM's two tile queues.
 __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* items, int nitems, unsigned int* queue)
@@ -181,6 +183,75 @@ __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* item
     const int local = (int)(b - items[idx].firstBlock);
     const int rowTiles = p.mnPad / 64;
     pack_tile(p, local % rowTiles, local / rowTiles, tile, esum);
+}
+
+// Energy partials of grouped members the quad kernel reads raw: the pack kernel's per-tile
+// sums (row r of a 64 x 64 tile: four quarters of 16 k, each summed in k order, the quarter
+// sums added in quarter order, FP64), so the energies equal the packed form's bit for bit,
+// computed without the LDS tile or any plane writes. Thread (row, quarter) issues its 16 loads
+// at once: rows adjacent in memory make each load a 512-byte run across the 64 threads of a
+// quarter; k-contiguous rows give each thread 128 contiguous bytes as eight 16-byte loads.
+// Out-of-range rows and k count as zero (clamped addresses, no branch around a load). The
+// first workgroup also resets the grouped GEMM's tile queue.
+__global__ __launch_bounds__(256) void energy_partial_grouped_kernel(const GroupPack* items, int nitems,
+                                                                     unsigned int* queue)
+{
+    __shared__ double qsum[256];
+    const long long b = blockIdx.x;
+    if (b == 0 && threadIdx.x < 2) queue[threadIdx.x] = 0u;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].firstBlock <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int idx = __builtin_amdgcn_readfirstlane(lo);
+    const PackArgs a = items[idx].a;
+    const int local = (int)(b - items[idx].firstBlock);
+    const int rowTiles = a.mnPad / 64;
+    const int rt = local % rowTiles, ct = local / rowTiles;
+    const int tid = threadIdx.x, row = tid & 63, qtr = tid >> 6;
+    const int r = rt * 64 + row, c0 = ct * 64 + qtr * 16;
+    const int rr = min(r, a.mn - 1);
+    const float2* src = reinterpret_cast<const float2*>(a.src);
+    float2 v[16];
+    if (a.mnContig) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = src[(size_t)min(c0 + t, a.kk - 1) * a.ld + rr];
+    } else if (c0 + 15 < a.kk) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            f4 w;
+            __builtin_memcpy(&w, src + (size_t)rr * a.ld + c0 + 2 * t, 16);
+            v[2 * t] = make_float2(w[0], w[1]);
+            v[2 * t + 1] = make_float2(w[2], w[3]);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = src[(size_t)rr * a.ld + min(c0 + t, a.kk - 1)];
+    }
+    double sq = 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const bool in = r < a.mn && c0 + t < a.kk;
+        const double x = in ? (double)v[t].x : 0.0, y = in ? (double)v[t].y : 0.0;
+        sq += x * x + y * y;
+    }
+    qsum[tid] = sq;
+    __syncthreads();
+    if (tid < 64)
+        a.energyPartial[(size_t)ct * a.mnPad + rt * 64 + tid] =
+            ((qsum[tid] + qsum[64 + tid]) + qsum[128 + tid]) + qsum[192 + tid];
+}
+
+hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
+                                         hipStream_t s)
+{
+    hipLaunchKernelGGL(energy_partial_grouped_kernel, dim3((unsigned)blocks), dim3(256), 0, s, items, nitems, queue);
+    return hipGetLastError();
 }
 
 hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,

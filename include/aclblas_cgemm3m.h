@@ -171,7 +171,9 @@ aclblasStatus_t aclblasCgemm3mGetWorkspaceSize(aclblasHandle_t handle, aclblasOp
                                                const aclblasCgemm3mConfig* config, size_t* bytes);
 
 /* For callers with many small independent problems: `count` problems in one persistent
- * launch (plus one packing launch and, if any problem is K-split, one merge launch);
+ * launch that reads their operands as stored (plus, if any problem has the rounding
+ * correction on, one energy launch before it and one recompute launch after it, and if any
+ * is K-split, one merge launch);
  * problems the autotuned table gives to the one-launch kernel run in one more launch per
  * form, on a second stream of the handle joined back before the call returns.
  * All arrays are HOST arrays of length count; A/B/C entries are device pointers.
@@ -187,7 +189,10 @@ aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count, 
 
 /* For the autotuner and callers that pin plans: aclblasCgemm3mGroupedBatched with one
  * config applied to every member (precision; tile and kSplit when nonzero; identity;
- * correction when nonzero, else on for members with k >= 1024 as in aclblasCgemm3m). */
+ * correction when nonzero, else on for members with k >= 1024 as in aclblasCgemm3m; variant 5
+ * puts every member on the one-launch kernel, variant 7 packs the persistent kernel's members
+ * into planes with one launch first instead of reading their operands as stored: the same
+ * bytes, kept for comparison). */
 aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
                                                const aclblasOperation_t* transb, const int* m, const int* n,
                                                const int* k, const aclblasComplex* alpha,

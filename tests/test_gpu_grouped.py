@@ -375,3 +375,28 @@ def test_create_reserves_default_workspace(gpu):
     free3 = torch.cuda.mem_get_info()[0]
     assert free3 == free2, (free2, free3)
     h.close()
+
+
+@pytest.mark.parametrize("pin", [dict(), dict(tile=128, k_split=3), dict(tile=64, k_split=1, identity=2),
+                                 dict(tile=64, k_split=2, correction=1)])
+def test_grouped_raw_reads_match_packed_planes(gpu, pin):
+    """The persistent kernel reads its members' operands as stored and forms the plane values in
+    registers; the form that packs them into planes with one launch first (variant 7) must give
+    the same bytes: same plane values, same MFMA order per accumulator (all 9 ops, uneven sizes,
+    K splits, both 3M identities, the correction on)."""
+    import torch
+
+    import cgemm3m
+
+    probs, host = make_problems(40, 91 + len(pin), True)
+    outs = []
+    for variant in (0, 7):
+        for h in host:
+            h["dC"].copy_(torch.from_numpy(h["C"]).cuda())
+        cgemm3m.grouped(gpu, probs, cfg=cgemm3m.config(variant=variant, **pin))
+        torch.cuda.synchronize()
+        outs.append([h["dC"].cpu().numpy().copy() for h in host])
+    for i, (r, p) in enumerate(zip(*outs)):
+        assert r.tobytes() == p.tobytes(), "member %d (%s%s %d x %d x %d) differs" % (
+            i, probs[i]["ta"], probs[i]["tb"], probs[i]["m"], probs[i]["n"], probs[i]["k"])
+    check_all(probs, host, "grouped-raw")
