@@ -19,7 +19,8 @@ kernel trace, launch gaps excluded. Two steps on the GPU:
 (shapes and candidate order); `pick` chooses, per class, the candidate with the smallest
 geometric mean of time / (that shape's best time) over the class's shapes, and also times the
 grouped classes with events. Without a profiler, `python -m cgemm3m.autotune events` times
-every candidate with events around the call (launch gaps included).
+every candidate with events around the call (launch gaps included); `grouped` re-times only
+the grouped classes and keeps the installed table's one-problem classes.
 
 The shapes: a few squares and tall/flat problems, plus this project's own seeded set of
 generated shapes (`generated_shapes`: every dimension 1..4096, all nine op pairs, some padded
@@ -31,6 +32,7 @@ import json
 import math
 import os
 import random
+import re
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -249,9 +251,20 @@ def pick_bins(measured):
     return out
 
 
+def read_single_entries(path=TABLE):
+    """The one-problem class entries of an existing table: [((mnLog2, kLog2), 0, (tile, parts))]."""
+    text = open(path).read()
+    body = text[text.index("kTunedTable[]"):]
+    body = body[:body.index("};")]
+    out = []
+    for a, b, g, t, p in re.findall(r"\{(\d+), (\d+), (\d+), (\d+), (\d+)\}", body):
+        if int(g) == 0 and int(t) != 0:
+            out.append(((int(a), int(b)), 0, (int(t), int(p))))
+    return out
+
+
 def read_grouped_entries(path=TABLE):
     """The grouped class entries of an existing table: [((mnLog2, kLog2), 1, (tile, parts))]."""
-    import re
 
     text = open(path).read()
     body = text[text.index("kTunedTable[]"):]
@@ -263,9 +276,14 @@ def read_grouped_entries(path=TABLE):
     return out
 
 
-def tune_grouped(h, torch, shapes, reps, count=64):
-    """Grouped classes: one grouped call of `count` identical problems per candidate, with
-    the member plan pinned through aclblasCgemm3mGroupedBatchedEx."""
+OP_PAIRS = [(a, b) for a in "NTC" for b in "NTC"]
+
+
+def tune_grouped(h, torch, shapes, reps, count=63):
+    """Grouped classes: one grouped call of `count` problems of one shape per candidate, their
+    op pairs cycling through all nine (the persistent kernel reads operands as stored, so its
+    speed depends on their layout), with the member plan pinned through
+    aclblasCgemm3mGroupedBatchedEx."""
     import cgemm3m
 
     timed = _timer(torch)
@@ -273,13 +291,15 @@ def tune_grouped(h, torch, shapes, reps, count=64):
     for (m, n, k) in shapes:
         keep = []
         probs = []
-        for _ in range(count):
+        for i in range(count):
+            ta, tb = OP_PAIRS[i % len(OP_PAIRS)]
             a = torch.rand(2 * m * k, device="cuda").view(torch.complex64)
             b = torch.rand(2 * k * n, device="cuda").view(torch.complex64)
             c = torch.zeros(m * n, dtype=torch.complex64, device="cuda")
             keep += [a, b, c]
-            probs.append(dict(ta="N", tb="N", m=m, n=n, k=k, alpha=1.0, A=cgemm3m.torch_ptr(a), lda=m,
-                              B=cgemm3m.torch_ptr(b), ldb=k, beta=0.0, C=cgemm3m.torch_ptr(c), ldc=m))
+            probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=1.0, A=cgemm3m.torch_ptr(a),
+                              lda=m if ta == "N" else k, B=cgemm3m.torch_ptr(b), ldb=k if tb == "N" else n,
+                              beta=0.0, C=cgemm3m.torch_ptr(c), ldc=m))
         res = {}
         for tile in (64, 128):
             for parts in (1, 2, 4):
@@ -329,8 +349,9 @@ def write_table(entries, path=TABLE):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["run", "pick", "events"])
-    ap.add_argument("paths", nargs="*", help="run: MANIFEST; pick: TRACE MANIFEST")
+    ap.add_argument("mode", choices=["run", "pick", "events", "grouped"])
+    ap.add_argument("paths", nargs="*", help="run: MANIFEST; pick: TRACE MANIFEST; grouped: none (re-times "
+                                             "the grouped classes, keeps the installed one-problem ones)")
     ap.add_argument("--shapes", default="all", choices=["builtin", "gen", "all"])
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--write", action="store_true")
@@ -350,6 +371,13 @@ def main():
     import cgemm3m
 
     h = cgemm3m.Handle()
+    if args.mode == "grouped":
+        grouped = GROUPED_SHAPES[::3] if args.quick else GROUPED_SHAPES
+        entries = read_single_entries() + tune_grouped(h, torch, grouped, args.reps)
+        if args.write:
+            print("wrote", write_table(entries, args.out))
+        h.close()
+        return
     shapes = shape_list(args.shapes)
     if args.quick:
         shapes = shapes[::10]

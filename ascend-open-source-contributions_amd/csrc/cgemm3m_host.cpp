@@ -347,8 +347,9 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     if (pl.correction) {
         const size_t outs = (size_t)m * (size_t)n;
         pl.bitmapWords = (long long)((outs + 63) / 64);
-        if (!add_region((size_t)pl.kTiles * pl.mPad * 8, total, pl.offEPA)) return false;
-        if (!add_region((size_t)pl.kTiles * pl.nPad * 8, total, pl.offEPB)) return false;
+        // energy partials: two halves per (k tile, row), the half-sum order
+        if (!add_region(2 * (size_t)pl.kTiles * pl.mPad * 8, total, pl.offEPA)) return false;
+        if (!add_region(2 * (size_t)pl.kTiles * pl.nPad * 8, total, pl.offEPB)) return false;
         if (!add_region((size_t)pl.mPad * 8, total, pl.offEA)) return false;
         if (!add_region((size_t)pl.nPad * 8, total, pl.offEB)) return false;
         if (!add_region((size_t)pl.bitmapWords * 8, total, pl.offBitmap)) return false;

@@ -154,6 +154,21 @@ __device__ __forceinline__ float2 reference_order_output(const SourceOperands& s
                          s.ldb, s.opB, i, j, s.k, alpha, beta, betaZero, c0);
 }
 
+// Squared row norm of op(A) (column norm of op(B)) for the screen below, in one fixed order
+// (the half-sum order) that the pack kernels and the grouped FP32 kernel follow, so the plans
+// built on them flag the same outputs. Per 16-deep k stage s and half h = 0, 1: the FP32 sum,
+// from zero, of the elements' squares (energy_acc) over k = 16 s + 8 f + 4 h + u in (f, u)
+// order (f = 0..1, u = 0..3);
+// per 64-deep k tile and half: the FP64 sum, from zero, of its four stages' sums in stage
+// order; per half, the tiles' partials summed from zero in tile order; the energy = half 0 +
+// half 1. (These are the k a lane of the grouped FP32 kernel holds: k = 4 (lane / 32) + u
+// within each 8, so that kernel forms the energy from the operands it loads, in registers,
+// at two FP32 FMAs per element.) The FP32 squares overflow to Inf for components past
+// about 1.8e19, which flags the outputs involved (recomputed in the slow, exact way); they
+// lose precision only below about 1e-19, where the products lose it too.
+__device__ __forceinline__ float energy_acc(float h, float x, float y) { return fmaf(y, y, fmaf(x, x, h)); }
+__device__ __forceinline__ float energy_acc(float h, float2 v) { return energy_acc(h, v.x, v.y); }
+
 // Rounding-correction screen for a finite output: flag it when its FP32 error estimate
 // corrScale * |op(A)_i|_2 * |op(B)_j|_2 exceeds corrTau * |c|. Both sides are non-negative,
 // so their squares are compared (ea, eb are the squared norms): no square root per output; a
@@ -178,10 +193,14 @@ __device__ __forceinline__ void flag_output(unsigned long long* bitmap, long lon
 // >= n never touched. An output flagged by the rounding-correction screen is not written
 // (the recompute pass reads C's input there); a non-finite one is recomputed by its lane in
 // the reference BLAS's order once the tile is stored.
-template <int TN, int TM, bool PARTIAL, int NACC = 3>
+template <int TN, int TM, bool PARTIAL, int NACC = 3, bool OWN_ENERGY = false>
 __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC][TN][TM], int part, int iBase,
-                                           int jBase)
+                                           int jBase, const double* eaIn = nullptr,
+                                           const __attribute__((address_space(3))) double* ebIn = nullptr)
 {
+    // OWN_ENERGY: the grouped kernel's own energies, eaIn[b] for row iBase + 32 b and ebIn[t]
+    // (in LDS, read with LDS loads) for column jBase + t; else eA / eB from the workspace. A
+    // compile-time choice, not a null test: an LDS address can be 0.
     static_assert(TN * TM * 16 <= 64, "one bit per output of the lane");
     // NACC == 3: the three products, combined by the 3M identity; NACC == 2: (re, im) as
     // accumulated by the four-product form.
@@ -221,7 +240,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC
         double ea[TM];
         if (g.correction) {
 #pragma unroll
-            for (int b = 0; b < TM; ++b) ea[b] = g.eA[iBase + b * 32];  // eA holds mPad rows
+            for (int b = 0; b < TM; ++b) ea[b] = OWN_ENERGY ? eaIn[b] : g.eA[iBase + b * 32];  // mPad rows
         }
 #pragma unroll
         for (int a = 0; a < TN; ++a) {
@@ -244,7 +263,10 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, f32x16 (&acc)[NACC
             }
             if (g.correction) {
 #pragma unroll
-                for (int r = 0; r < 16; ++r) eb[r] = g.eB[jBase + a * 32 + 8 * (r >> 2) + (r & 3)];  // nPad columns
+                for (int r = 0; r < 16; ++r) {
+                    const int t = a * 32 + 8 * (r >> 2) + (r & 3);
+                    eb[r] = OWN_ENERGY ? ebIn[t] : g.eB[jBase + t];  // nPad columns
+                }
             }
 #pragma unroll
             for (int b = 0; b < TM; ++b)

@@ -335,7 +335,7 @@ struct RawSet {
     rawf2 a[2][2][4], b[2][2][4];  // [k fragment][32-row / 32-column fragment][k slot]
 };
 
-template <bool PARTIAL, int ID>
+template <int ID>
 __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int tmI, int tnI)
 {
     constexpr int TM = 2, TN = 2, KFD = 2, NSET = 2;
@@ -463,11 +463,24 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
         if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? bSign * im : fmaf(bSign, im, re));
         return q == 0 ? re : (q == 1 ? fmaf(bSign, im, -re) : fmaf(bSign, im, re));  // (Br, Bi-Br, Br+Bi)
     };
+    // Row and column energies for the rounding-correction screen, formed from the loaded
+    // values in the half-sum order of energy_acc: per stage, the lane's half h sums its 8 k of
+    // each row (column) in FP32 (H); T (FP64) sums the stages' H of a 64-deep k tile from
+    // zero; R sums the tiles' T from zero in tile order; the two halves' R are added after the
+    // loop. Only fused tiles of members with the correction on form them: a uniform branch
+    // after each stage's MFMAs (VALU work only; a second copy of the k loops instead would
+    // make the compiler wait for all loads in flight at every stage of that copy).
+    double T[TM + TN], R[TM + TN];
+#pragma unroll
+    for (int q = 0; q < TM + TN; ++q) T[q] = R[q] = 0.0;
+    const bool energy = g.parts == 1 && g.correction && g.energyInline;
+
     // MASK: k past K contributes zeros (both operands, so an Inf or NaN read from the clamped
-    // address adds nothing).
+    // address adds nothing), to the products and to the energies.
     auto mfma = [&](const RawSet<ID>& S, int st, auto maskTag) {
         constexpr bool MASK = decltype(maskTag)::value;
         const int sk = (kf0 + st * KFD) * 8;
+        const int pos = (kf0 / KFD + st) & 3;  // stage within its 64-deep k tile
 #pragma unroll
         for (int kfl = 0; kfl < KFD; ++kfl)
 #pragma unroll
@@ -485,6 +498,32 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
                             acc[q][a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(x, y, acc[q][a][b], 0, 0, 0);
                         }
             }
+        if (energy) {  // uniform; VALU only, after the stage's MFMAs are issued
+            float H[TM + TN];
+#pragma unroll
+            for (int q = 0; q < TM + TN; ++q) H[q] = 0.0f;
+#pragma unroll
+            for (int kfl = 0; kfl < KFD; ++kfl)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const bool kin = !MASK || sk + kfl * 8 + 4 * half + s < K;
+#pragma unroll
+                    for (int b = 0; b < TM; ++b) {
+                        const rawf2 v = S.a[kfl][b][s];
+                        H[b] = energy_acc(H[b], kin ? v.x : 0.0f, kin ? v.y : 0.0f);
+                    }
+#pragma unroll
+                    for (int a = 0; a < TN; ++a) {
+                        const rawf2 v = S.b[kfl][a][s];
+                        H[TM + a] = energy_acc(H[TM + a], kin ? v.x : 0.0f, kin ? v.y : 0.0f);
+                    }
+                }
+#pragma unroll
+            for (int q = 0; q < TM + TN; ++q) {
+                T[q] = (pos == 0 ? 0.0 : T[q]) + (double)H[q];
+                R[q] = pos == 3 ? R[q] + T[q] : R[q];
+            }
+        }
     };
     typedef std::integral_constant<bool, false> Off;
     typedef std::integral_constant<bool, true> On;
@@ -527,14 +566,39 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
             ring(Off(), Off());
     }
     for (int t = nFull; t < nst; ++t) {
-        RawSet<ID> T;
-        loadClamped(T, t);
-        mfma(T, t, On());
+        RawSet<ID> Tl;
+        loadClamped(Tl, t);
+        mfma(Tl, t, On());
     }
 
-    const int iBase = i0 + (lane & 31);
-    const int jBase = j0 + 4 * half;
-    store_tile<TN, TM, PARTIAL, 3>(g, acc, part, iBase, jBase);
+    const int iBase = i0 + (lane & 31), jBase = j0 + 4 * half;
+    if (g.parts > 1) {  // K split: the merge screens, from energies computed before this kernel
+        store_tile<TN, TM, true, 3>(g, acc, part, iBase, jBase);
+    } else if (energy) {
+        __shared__ double ebShared[4][64];  // per wave: its 64 columns' energies
+        const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        if ((nst & 3) != 0) {  // the last k tile is partial: add its partial sum
+#pragma unroll
+            for (int q = 0; q < TM + TN; ++q) R[q] += T[q];
+        }
+        double ea[TM];
+#pragma unroll
+        for (int b = 0; b < TM; ++b) ea[b] = R[b] + __shfl_xor(R[b], 32);  // half 0 + half 1 (commutes)
+#pragma unroll
+        for (int a = 0; a < TN; ++a) {
+            const double e = R[TM + a] + __shfl_xor(R[TM + a], 32);
+            if (half == 0) ebShared[wave][32 * a + lane] = e;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        store_tile<TN, TM, false, 3, true>(g, acc, part, iBase, jBase, ea,
+                                           (const __attribute__((address_space(3))) double*)&ebShared[wave][4 * half]);
+        // the next tile of this wave rewrites ebShared only after these reads (in order per wave)
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        store_tile<TN, TM, false, 3>(g, acc, part, iBase, jBase);
+    }
 }
 
 // Grouped batch on 64 x 64 wave tiles: a persistent grid of four-wave workgroups pulls quads
@@ -566,18 +630,7 @@ __global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_quad_kernel(const G
         const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
         const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
         if constexpr (RAW) {
-            const bool std3m = __builtin_amdgcn_readfirstlane(g.identity) == ID_STANDARD;
-            if (g.parts > 1) {
-                if (std3m)
-                    f32_tile_raw64<true, ID_STANDARD>(g, part, tm, tn);
-                else
-                    f32_tile_raw64<true, ID_FUSED>(g, part, tm, tn);
-            } else {
-                if (std3m)
-                    f32_tile_raw64<false, ID_STANDARD>(g, part, tm, tn);
-                else
-                    f32_tile_raw64<false, ID_FUSED>(g, part, tm, tn);
-            }
+            f32_tile_raw64<ID_FUSED>(g, part, tm, tn);  // the host sends fused-identity batches only
         } else if (g.parts > 1) {
             f32_tile_direct<64, 64, true, 1, 1, false, C3M_DIRECT_KFD, C3M_DIRECT_NSET, true>(g, part, tm, tn);
         } else {

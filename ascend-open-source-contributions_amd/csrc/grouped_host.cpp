@@ -180,7 +180,8 @@ struct GroupedLayout {
     std::vector<Plan> smallPlans;
     size_t descBytes = 0, offQueue = 0, total = 0;
     size_t nProb = 0, nPack = 0, nQuad = 0, nMerge = 0;
-    size_t nCorr = 0;               // packed members with the rounding correction on
+    size_t nCorr = 0;               // quad-kernel members with the rounding correction on
+    size_t nCorrSplit = 0;          // those of them with a K split (energies before the merge)
     size_t energyBytes = 0;         // descriptor bytes of their energy items
     long long packBlocks = 0;
     int maxMergeOutputs = 0;
@@ -190,8 +191,9 @@ struct GroupedLayout {
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 // Workspace of a member the quad kernel reads raw: no planes, only the K-split partial planes
-// and, with the correction on, the energy partials (from an energy-only pass of the pack
-// kernel, so the energies equal the packed form's bit for bit), the energies and the bitmap.
+// and, with the correction on, the bitmap and counter; a K-split member also keeps its energy
+// partials (from the energy-only pass, two halves per k tile) and energies, which the merge
+// reads. A fused member's energies are formed by the quad kernel itself, in the same order.
 static void raw_member_workspace(Plan& pl)
 {
     size_t total = 0;
@@ -204,10 +206,12 @@ static void raw_member_workspace(Plan& pl)
     pl.aPlaneBytes = pl.bPlaneBytes = 0;  // no planes: pack items of this member write none
     if (pl.parts > 1) pl.offPartial = region(pl.partStride * 4 * (size_t)pl.parts);
     if (pl.correction) {
-        pl.offEPA = region((size_t)pl.kTiles * pl.mPad * 8);
-        pl.offEPB = region((size_t)pl.kTiles * pl.nPad * 8);
-        pl.offEA = region((size_t)pl.mPad * 8);
-        pl.offEB = region((size_t)pl.nPad * 8);
+        if (pl.parts > 1) {
+            pl.offEPA = region(2 * (size_t)pl.kTiles * pl.mPad * 8);
+            pl.offEPB = region(2 * (size_t)pl.kTiles * pl.nPad * 8);
+            pl.offEA = region((size_t)pl.mPad * 8);
+            pl.offEB = region((size_t)pl.nPad * 8);
+        }
         pl.offBitmap = region((size_t)pl.bitmapWords * 8);
         pl.offCounter = region(256);
     }
@@ -258,8 +262,9 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
                                     aclblasComplex* const* C, const int* ldc, bool withPointers,
                                     const aclblasCgemm3mConfig* over, GroupedLayout& L, const char* op)
 {
-    // variant 7 keeps the earlier form: members packed into planes by one launch first
-    L.raw = !(over != nullptr && over->variant == 7);
+    // variant 7 keeps the earlier form: members packed into planes by one launch first; so
+    // does a batch on the standard identity (the raw kernel forms the fused identity's planes)
+    L.raw = !(over != nullptr && (over->variant == 7 || over->identity == ID_STANDARD));
     // The raw quad kernel addresses a tile's elements by 32-bit byte offsets (at most 68 leading
     // dimensions of 8 bytes from the tile's base); wider leading dimensions take the packed form.
     for (int i = 0; i < count && withPointers; ++i)
@@ -310,11 +315,14 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             L.maxMergeOutputs = std::max<long long>(L.maxMergeOutputs, (long long)L.probs[i].m * L.probs[i].n);
         }
         if (pl.correction) ++L.nCorr;
+        if (pl.correction && pl.parts > 1) ++L.nCorrSplit;
     }
     L.nQuad += (loose + 3) / 4;
     L.nProb = L.probs.size();
-    L.nPack = 2 * (L.raw ? L.nCorr : L.nProb);  // raw: energy-only items of the corrected members
-    L.energyBytes = L.nCorr * sizeof(EnergyArgs);
+    // raw: energy-only items of the corrected K-split members; energy items: one per corrected
+    // member (raw: bitmap clears, plus the K-split members' energy sums)
+    L.nPack = 2 * (L.raw ? L.nCorrSplit : L.nProb);
+    L.energyBytes = (L.nCorr + (L.raw ? L.nCorrSplit : 0)) * sizeof(EnergyArgs);
     L.descBytes = align256(L.nProb * sizeof(GroupProblem)) + align256(L.nPack * sizeof(GroupPack)) +
                   align256(4 * L.nQuad * sizeof(GroupTile)) + align256(L.nMerge * sizeof(GroupMerge)) +
                   align256(L.small.size() * sizeof(GroupSmall)) + align256(L.energyBytes) +
@@ -347,11 +355,18 @@ extern "C" aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasH
     if (precision != ACLBLAS_CGEMM3M_FP32) return ACLBLAS_STATUS_NOT_SUPPORTED;
     if (ensure_device(handle) != ACLBLAS_STATUS_SUCCESS) handle->cus = 0;
     std::vector<aclblasComplex> one(count, aclblasComplex{1.0f, 0.0f}), zero(count, aclblasComplex{0.0f, 0.0f});
-    GroupedLayout L;
-    const aclblasStatus_t st = build_layout(handle, count, transa, transb, m, n, k, one.data(), nullptr, nullptr,
-                                            nullptr, nullptr, zero.data(), nullptr, nullptr, false, nullptr, L, op);
+    // The larger of the two forms' needs: the call reads the operands as stored unless a
+    // leading dimension is too wide for that kernel's offsets, which only the call sees.
+    GroupedLayout L, Lp;
+    aclblasCgemm3mConfig packed{};
+    packed.variant = 7;
+    aclblasStatus_t st = build_layout(handle, count, transa, transb, m, n, k, one.data(), nullptr, nullptr, nullptr,
+                                      nullptr, zero.data(), nullptr, nullptr, false, nullptr, L, op);
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
-    *bytes = L.total;
+    st = build_layout(handle, count, transa, transb, m, n, k, one.data(), nullptr, nullptr, nullptr, nullptr,
+                      zero.data(), nullptr, nullptr, false, &packed, Lp, op);
+    if (st != ACLBLAS_STATUS_SUCCESS) return st;
+    *bytes = std::max(L.total, Lp.total);
     return ACLBLAS_STATUS_SUCCESS;
 }
 
@@ -415,11 +430,15 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         };
         std::vector<Quad> quads;
         std::vector<Cost> loose;
+        unsigned int* queue0 = L.nProb > 0 ? reinterpret_cast<unsigned int*>(ws + L.offQueue) : nullptr;
         quads.reserve(L.nQuad);
-        // members with the rounding correction: energies (after the pack) and recompute (last)
-        std::vector<EnergyArgs> energies;  // from the pack's (or the energy-only pass's) partials
+        // members with the rounding correction: energies and recompute (last). Packed form:
+        // one energy item per member (sums of the pack's partials, bitmap clear). Raw form:
+        // bitmap clears for every such member before the quad kernel (the first item also
+        // clears the tile queue), and energy sums of the K-split ones, whose merge screens.
+        std::vector<EnergyArgs> energies, sums;
         std::vector<RecomputeArgs> recomputes;
-        long long maxEnergyWork = 0, maxWords = 0;
+        long long maxEnergyWork = 0, maxSumWork = 0, maxWords = 0;
         long long packBlocks = 0;
         size_t nm = 0, npk = 0;
         for (size_t i = 0; i < L.nProb; ++i) {
@@ -450,7 +469,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 a.energyPartial =
                     pl.correction ? reinterpret_cast<double*>(pws + (isA ? pl.offEPA : pl.offEPB)) : nullptr;
                 a.scaleExp = nullptr;
-                if (L.raw && !pl.correction) continue;  // read as stored by the quad kernel
+                if (L.raw && !(pl.correction && pl.parts > 1)) continue;  // read as stored by the quad kernel
                 pk[npk].a = a;
                 pk[npk].firstBlock = packBlocks;
                 ++npk;
@@ -486,15 +505,30 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             if (pl.correction) {
                 const double amag =
                     std::sqrt((double)p.alpha.real * p.alpha.real + (double)p.alpha.imag * p.alpha.imag);
-                g.eA = reinterpret_cast<const double*>(pws + pl.offEA);
-                g.eB = reinterpret_cast<const double*>(pws + pl.offEB);
+                g.energyInline = L.raw && pl.parts == 1 ? 1 : 0;
+                if (!g.energyInline) {
+                    g.eA = reinterpret_cast<const double*>(pws + pl.offEA);
+                    g.eB = reinterpret_cast<const double*>(pws + pl.offEB);
+                }
                 g.corrScale = amag * 0x1p-24;
                 g.corrTau = 0x1p-8;
                 g.bitmap = reinterpret_cast<unsigned long long*>(pws + pl.offBitmap);
-                energies.push_back(EnergyArgs{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles,
-                                              const_cast<double*>(g.eA), const_cast<double*>(g.eB), g.bitmap,
-                                              pl.bitmapWords, nullptr});
-                maxEnergyWork = std::max<long long>({maxEnergyWork, pl.mPad, pl.nPad, pl.bitmapWords});
+                if (!L.raw) {
+                    energies.push_back(EnergyArgs{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles,
+                                                  const_cast<double*>(g.eA), const_cast<double*>(g.eB), g.bitmap,
+                                                  pl.bitmapWords, nullptr});
+                    maxEnergyWork = std::max<long long>({maxEnergyWork, pl.mPad, pl.nPad, pl.bitmapWords});
+                } else {
+                    energies.push_back(EnergyArgs{nullptr, 0, nullptr, 0, 0, nullptr, nullptr, g.bitmap,
+                                                  pl.bitmapWords, energies.empty() ? queue0 : nullptr});
+                    maxEnergyWork = std::max<long long>(maxEnergyWork, pl.bitmapWords);
+                    if (pl.parts > 1) {
+                        sums.push_back(EnergyArgs{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles,
+                                                  const_cast<double*>(g.eA), const_cast<double*>(g.eB), nullptr, 0,
+                                                  nullptr});
+                        maxSumWork = std::max<long long>({maxSumWork, pl.mPad, pl.nPad});
+                    }
+                }
                 RecomputeArgs r{};
                 r.A = p.A;
                 r.lda = p.lda;
@@ -619,14 +653,18 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             }
         const size_t offEnergy = offSmall + align256(L.small.size() * sizeof(GroupSmall));
         const size_t offRecompute = offEnergy + align256(L.energyBytes);
-        unsigned int* queue = L.nProb > 0 ? reinterpret_cast<unsigned int*>(ws + L.offQueue) : nullptr;
-        if (energies.size() != L.nCorr || recomputes.size() != L.nCorr || npk != L.nPack) {
+        unsigned int* queue = queue0;
+        if (energies.size() != L.nCorr || recomputes.size() != L.nCorr || npk != L.nPack ||
+            sums.size() != (L.raw ? L.nCorrSplit : 0)) {
             log_error(op, "internal error: %zu corrected members and %zu pack items laid out, %zu and %zu counted",
                       energies.size(), npk, L.nCorr, L.nPack);
             return ACLBLAS_STATUS_INTERNAL_ERROR;
         }
         if (L.nCorr > 0) {
-            std::memcpy(blob.data() + offEnergy, energies.data(), L.energyBytes);
+            std::memcpy(blob.data() + offEnergy, energies.data(), energies.size() * sizeof(EnergyArgs));
+            if (!sums.empty())
+                std::memcpy(blob.data() + offEnergy + energies.size() * sizeof(EnergyArgs), sums.data(),
+                            sums.size() * sizeof(EnergyArgs));
             std::memcpy(blob.data() + offRecompute, recomputes.data(), L.nCorr * sizeof(RecomputeArgs));
         }
 
@@ -636,12 +674,19 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             log_error(op, "descriptor staging failed: %s", hipGetErrorString(e));
             return ACLBLAS_STATUS_ALLOC_FAILED;
         }
-        // The one-launch members share nothing with the packed ones, so when there are both
-        // they run on the handle's second stream, forked after the descriptor upload and joined
-        // before the call returns (under hipGraph capture this becomes two parallel branches).
-        // Their kernel times are not in the stage timings.
+        // Work that the quad kernel does not wait for runs on the handle's second stream, forked
+        // after the descriptor upload and joined before the merge (under hipGraph capture this
+        // becomes two parallel branches): the one-launch members, which share nothing with
+        // the quad-kernel members, and, in the raw form, the energies of the K-split corrected
+        // members, which only their merge reads. Side-stream kernel times are not in the
+        // stage timings.
         const GroupSmall* dSmall = reinterpret_cast<const GroupSmall*>(dDesc + offSmall);
-        const bool fork = !L.small.empty() && L.nProb > 0;
+        const GroupProblem* dProb = reinterpret_cast<const GroupProblem*>(dDesc);
+        const GroupPack* dPack = reinterpret_cast<const GroupPack*>(dDesc + offPack);
+        const GroupTile* dTiles = reinterpret_cast<const GroupTile*>(dDesc + offTiles);
+        const GroupMerge* dMerge = reinterpret_cast<const GroupMerge*>(dDesc + offMerge);
+        const EnergyArgs* dEnergy = reinterpret_cast<const EnergyArgs*>(dDesc + offEnergy);
+        const bool fork = L.nProb > 0 && (!L.small.empty() || !sums.empty());
         hipStream_t ss = s;
         if (fork) {
             RelaxedCapture relaxed;
@@ -661,7 +706,14 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             e = hipEventRecord(handle->forkEv, s);
             if (e == hipSuccess) e = hipStreamWaitEvent(ss, handle->forkEv, 0);
         }
-        for (int form = 0; form < 2 && e == hipSuccess; ++form) {
+        if (e == hipSuccess && L.raw && !sums.empty()) {
+            e = launch_energy_partial_grouped(dPack, (int)L.nPack, packBlocks, ss);
+            if (e == hipSuccess) e = launch_energy_grouped(dEnergy + energies.size(), (int)sums.size(), maxSumWork, ss);
+        }
+        // single-wave form first: it is short, and a launch that follows the quad kernel's
+        // start finds every CU held by it until the queue empties
+        for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+            const int form = 1 - f;
             if (nSmall[form] == 0) continue;
             if (fork) {
                 e = launch_gemm_small_grouped(dSmall + (form == 0 ? 0 : nSmall[0]), (int)nSmall[form],
@@ -673,23 +725,16 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             }
         }
         if (fork && e == hipSuccess) e = hipEventRecord(handle->joinEv, ss);
-        const GroupProblem* dProb = reinterpret_cast<const GroupProblem*>(dDesc);
-        const GroupPack* dPack = reinterpret_cast<const GroupPack*>(dDesc + offPack);
-        const GroupTile* dTiles = reinterpret_cast<const GroupTile*>(dDesc + offTiles);
-        const GroupMerge* dMerge = reinterpret_cast<const GroupMerge*>(dDesc + offMerge);
-        if (e == hipSuccess && L.nPack > 0) {
+        if (e == hipSuccess && !L.raw && L.nPack > 0) {
             StageTimer t(handle, ST_PACK);
-            if (L.raw)  // energy partials of the corrected members only
-                e = launch_energy_partial_grouped(dPack, (int)L.nPack, packBlocks, queue, s);
-            else
-                e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);  // also clears the queue
-        } else if (e == hipSuccess && L.nProb > 0) {
+            e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);  // also clears the queue
+        } else if (e == hipSuccess && L.nProb > 0 && !(L.raw && L.nCorr > 0)) {
             e = launch_fill_u64(reinterpret_cast<unsigned long long*>(queue), 1, s);  // the tile queue
         }
         if (e == hipSuccess && L.nCorr > 0) {
+            // packed form: energy sums and bitmap clears; raw form: bitmap clears and the queue
             StageTimer t(handle, ST_ENERGY);
-            e = launch_energy_grouped(reinterpret_cast<const EnergyArgs*>(dDesc + offEnergy), (int)L.nCorr,
-                                      maxEnergyWork, s);
+            e = launch_energy_grouped(dEnergy, (int)L.nCorr, maxEnergyWork, s);
         }
         if (e == hipSuccess && L.nProb > 0) {
             StageTimer t(handle, ST_GEMM);
@@ -698,6 +743,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 e = launch_gemm_f32_grouped_quads(dProb, dTiles, (int)quads.size(), queue,
                                                   std::min<int>((int)quads.size(), cus), L.raw, s);
         }
+        if (fork && e == hipSuccess) e = hipStreamWaitEvent(s, handle->joinEv, 0);
         if (e == hipSuccess && nm > 0) {
             StageTimer t(handle, ST_MERGE);
             e = launch_merge_grouped(dMerge, (int)nm, L.maxMergeOutputs, s);
@@ -707,7 +753,6 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
             e = launch_recompute_grouped(reinterpret_cast<const RecomputeArgs*>(dDesc + offRecompute), (int)L.nCorr,
                                          maxWords, s);
         }
-        if (fork && e == hipSuccess) e = hipStreamWaitEvent(s, handle->joinEv, 0);
         // the slot is free for reuse once this call's kernels are done (a captured slot is
         // never reused)
         _aclblas_handle::Staged& used = handle->staged[slot];

@@ -109,6 +109,8 @@ struct GemmArgs {
     // rounding correction (fused path only); a flagged output is left unwritten (C keeps its
     // input, which the recompute reads when beta != 0)
     int correction;
+    int energyInline;                       // grouped FP32 kernel, reading the operands as stored:
+                                            // the energies are formed from its loads (eA, eB unused)
     const double* eA; const double* eB;
     double corrScale;                       // |alpha| * u  (error estimate = corrScale * |a_i||b_j|)
     double corrTau;                         // flag when estimate > corrTau * |c|
@@ -144,6 +146,8 @@ struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/b
     float alphaR, alphaI, betaR, betaI; int betaZero;
     void* C; long long ldc;
     int correction;
+    int energyInline;                       // grouped FP32 kernel, reading the operands as stored:
+                                            // the energies are formed from its loads (eA, eB unused)
     const double* eA; const double* eB; double corrScale, corrTau;
     unsigned long long* bitmap;
     SourceOperands src; // non-finite outputs are recomputed from these

@@ -25,8 +25,7 @@ hipError_t launch_residual(const ResidualArgs& a, int blocksM, int blocksN, hipS
 hipError_t launch_fill_u64(unsigned long long* p, long long words, hipStream_t s);
 hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
                                hipStream_t s);
-hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
-                                         hipStream_t s);
+hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, hipStream_t s);
 hipError_t launch_gemm_f32_grouped_quads(const GroupProblem* probs, const GroupTile* tiles, int nquads,
                                          unsigned int* queue, int blocks, bool raw, hipStream_t s);
 hipError_t launch_merge_grouped(const GroupMerge* items, int nitems, int maxOutputs, hipStream_t s);

@@ -19,7 +19,7 @@
 
 namespace c3m {
 
-__device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, float2 (*tile)[65], double* esum)
+__device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, float2 (*tile)[65])
 {
     const int tid = threadIdx.x;
     const int r0 = rt * 64, c0 = ct * 64;
@@ -49,19 +49,24 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
     __syncthreads();
 
     if (a.energyPartial != nullptr) {
-        // row = tid % 64 sums its quarter of the tile's 64 k in order, then the four quarter
-        // sums are added in quarter order: a fixed order, so the result is reproducible.
-        const int row = tid & 63, qtr = tid >> 6;
-        double sq = 0.0;
-        for (int cl = qtr * 16; cl < qtr * 16 + 16; ++cl) {
-            const float2 w = tile[row][cl];
-            sq += (double)w.x * w.x + (double)w.y * w.y;
+        // Energy partials in the half-sum order (energy_acc): thread (row, h) sums each stage's
+        // k = 16 s + 8 f + 4 h + u in (f, u) order in FP32, the stages in FP64; partial h of k
+        // tile ct goes to energyPartial[h][ct][row].
+        if (tid < 128) {
+            const int row = tid & 63, h = tid >> 6;
+            const int kTiles = (a.kPad + 63) / 64;
+            double t = 0.0;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                float hs = 0.0f;
+#pragma unroll
+                for (int f = 0; f < 2; ++f)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) hs = energy_acc(hs, tile[row][16 * s + 8 * f + 4 * h + u]);
+                t += (double)hs;
+            }
+            a.energyPartial[((size_t)h * kTiles + ct) * a.mnPad + r0 + row] = t;
         }
-        esum[qtr * 64 + row] = sq;
-        __syncthreads();
-        if (tid < 64)
-            a.energyPartial[(size_t)ct * a.mnPad + r0 + tid] =
-                ((esum[tid] + esum[64 + tid]) + esum[128 + tid]) + esum[192 + tid];
     }
 
     if (a.planes == nullptr) return;  // energy partials only (grouped members read raw)
@@ -137,12 +142,11 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
 __global__ __launch_bounds__(256) void pack_kernel(PackArgs a, PackArgs b, int blocksA)
 {
     __shared__ float2 tile[64][65];
-    __shared__ double esum[256];
     const bool isA = (int)blockIdx.x < blocksA;
     const PackArgs& p = isA ? a : b;
     const int local = isA ? blockIdx.x : blockIdx.x - blocksA;
     const int rowTiles = p.mnPad / 64;
-    pack_tile(p, local % rowTiles, local / rowTiles, tile, esum);
+    pack_tile(p, local % rowTiles, local / rowTiles, tile);
 }
 
 static int pack_blocks(const PackArgs& a) { return (a.mnPad / 64) * ((a.kPad + 63) / 64); }
@@ -167,7 +171,6 @@ hipError_t launch_pack(const PackArgs& a, hipStream_t s)
 __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* items, int nitems, unsigned int* queue)
 {
     __shared__ float2 tile[64][65];
-    __shared__ double esum[256];
     const long long b = blockIdx.x;
     if (b == 0 && threadIdx.x < 2) queue[threadIdx.x] = 0u;
     int lo = 0, hi = nitems - 1;
@@ -182,23 +185,19 @@ __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* item
     const PackArgs p = items[idx].a;
     const int local = (int)(b - items[idx].firstBlock);
     const int rowTiles = p.mnPad / 64;
-    pack_tile(p, local % rowTiles, local / rowTiles, tile, esum);
+    pack_tile(p, local % rowTiles, local / rowTiles, tile);
 }
 
-// Energy partials of grouped members the quad kernel reads raw: the pack kernel's per-tile
-// sums (row r of a 64 x 64 tile: four quarters of 16 k, each summed in k order, the quarter
-// sums added in quarter order, FP64), so the energies equal the packed form's bit for bit,
-// computed without the LDS tile or any plane writes. Thread (row, quarter) issues its 16 loads
-// at once: rows adjacent in memory make each load a 512-byte run across the 64 threads of a
-// quarter; k-contiguous rows give each thread 128 contiguous bytes as eight 16-byte loads.
-// Out-of-range rows and k count as zero (clamped addresses, no branch around a load). The
-// first workgroup also resets the grouped GEMM's tile queue.
-__global__ __launch_bounds__(256) void energy_partial_grouped_kernel(const GroupPack* items, int nitems,
-                                                                     unsigned int* queue)
+// Energy partials of grouped members whose energies come before the product kernel (K-split
+// members, whose screen runs in the merge): the pack kernel's partials (thread (row, h): 32
+// elements of the 64 x 64 tile in the half-sum order of energy_acc), computed without the LDS tile or
+// any plane writes. A thread issues its 32 loads at once: rows adjacent in memory make each
+// load a 512-byte run across the 64 threads of a half; k-contiguous rows give each thread
+// four runs of 32 contiguous bytes per 16 k. Out-of-range rows and k count as zero (clamped
+// addresses, no branch around a load). 128 threads per tile.
+__global__ __launch_bounds__(128) void energy_partial_grouped_kernel(const GroupPack* items, int nitems)
 {
-    __shared__ double qsum[256];
     const long long b = blockIdx.x;
-    if (b == 0 && threadIdx.x < 2) queue[threadIdx.x] = 0u;
     int lo = 0, hi = nitems - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -212,45 +211,53 @@ __global__ __launch_bounds__(256) void energy_partial_grouped_kernel(const Group
     const int local = (int)(b - items[idx].firstBlock);
     const int rowTiles = a.mnPad / 64;
     const int rt = local % rowTiles, ct = local / rowTiles;
-    const int tid = threadIdx.x, row = tid & 63, qtr = tid >> 6;
-    const int r = rt * 64 + row, c0 = ct * 64 + qtr * 16;
+    const int tid = threadIdx.x, row = tid & 63, h = tid >> 6;
+    const int r = rt * 64 + row, c0 = ct * 64;
     const int rr = min(r, a.mn - 1);
     const float2* src = reinterpret_cast<const float2*>(a.src);
-    float2 v[16];
+    float2 v[32];  // element (s, f, u) at 8 s + 4 f + u: k = c0 + 16 s + 8 f + 4 h + u
     if (a.mnContig) {
 #pragma unroll
-        for (int t = 0; t < 16; ++t) v[t] = src[(size_t)min(c0 + t, a.kk - 1) * a.ld + rr];
-    } else if (c0 + 15 < a.kk) {
+        for (int e = 0; e < 32; ++e) {
+            const int k = c0 + 16 * (e >> 3) + 8 * ((e >> 2) & 1) + 4 * h + (e & 3);
+            v[e] = src[(size_t)min(k, a.kk - 1) * a.ld + rr];
+        }
+    } else if (c0 + 63 < a.kk) {
         typedef float f4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
+        for (int e = 0; e < 32; e += 2) {
+            const int k = c0 + 16 * (e >> 3) + 8 * ((e >> 2) & 1) + 4 * h + (e & 3);
             f4 w;
-            __builtin_memcpy(&w, src + (size_t)rr * a.ld + c0 + 2 * t, 16);
-            v[2 * t] = make_float2(w[0], w[1]);
-            v[2 * t + 1] = make_float2(w[2], w[3]);
+            __builtin_memcpy(&w, src + (size_t)rr * a.ld + k, 16);
+            v[e] = make_float2(w[0], w[1]);
+            v[e + 1] = make_float2(w[2], w[3]);
         }
     } else {
 #pragma unroll
-        for (int t = 0; t < 16; ++t) v[t] = src[(size_t)rr * a.ld + min(c0 + t, a.kk - 1)];
+        for (int e = 0; e < 32; ++e) {
+            const int k = c0 + 16 * (e >> 3) + 8 * ((e >> 2) & 1) + 4 * h + (e & 3);
+            v[e] = src[(size_t)rr * a.ld + min(k, a.kk - 1)];
+        }
     }
-    double sq = 0.0;
+    double t = 0.0;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        const bool in = r < a.mn && c0 + t < a.kk;
-        const double x = in ? (double)v[t].x : 0.0, y = in ? (double)v[t].y : 0.0;
-        sq += x * x + y * y;
+    for (int s = 0; s < 4; ++s) {
+        float hs = 0.0f;
+#pragma unroll
+        for (int e = 8 * s; e < 8 * s + 8; ++e) {
+            const int k = c0 + 16 * (e >> 3) + 8 * ((e >> 2) & 1) + 4 * h + (e & 3);
+            const bool in = r < a.mn && k < a.kk;
+            hs = energy_acc(hs, in ? v[e] : make_float2(0.0f, 0.0f));
+        }
+        t += (double)hs;
     }
-    qsum[tid] = sq;
-    __syncthreads();
-    if (tid < 64)
-        a.energyPartial[(size_t)ct * a.mnPad + rt * 64 + tid] =
-            ((qsum[tid] + qsum[64 + tid]) + qsum[128 + tid]) + qsum[192 + tid];
+    const int kTiles = (a.kPad + 63) / 64;
+    a.energyPartial[((size_t)h * kTiles + ct) * a.mnPad + r] = t;
 }
 
-hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
-                                         hipStream_t s)
+hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, hipStream_t s)
 {
-    hipLaunchKernelGGL(energy_partial_grouped_kernel, dim3((unsigned)blocks), dim3(256), 0, s, items, nitems, queue);
+    hipLaunchKernelGGL(energy_partial_grouped_kernel, dim3((unsigned)blocks), dim3(128), 0, s, items, nitems);
     return hipGetLastError();
 }
 
@@ -387,7 +394,8 @@ hipError_t launch_scale_summary(const ScaleSummaryArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-// Row energies e[r] = sum over k tiles (index order) of the pack kernel's partials; also
+// Row energies e[r] = (sum over k tiles, in index order, of the pack kernel's half-0 partials)
+// + (the same for half 1), the half-sum order of energy_acc; also
 // clears the correction bitmap and the flagged-output counter for this call.
 // sum over kt of p[kt * ld] in index order; eight loads issued ahead of their adds
 __device__ __forceinline__ double ordered_sum(const double* p, size_t ld, int n)
@@ -409,8 +417,12 @@ __device__ __forceinline__ void energy_body(const EnergyArgs& a)
 {
     const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long stride = (long long)gridDim.x * 256;
-    for (long long r = t; r < a.mnPadA; r += stride) a.eA[r] = ordered_sum(a.partialA + r, a.mnPadA, a.kTiles);
-    for (long long r = t; r < a.mnPadB; r += stride) a.eB[r] = ordered_sum(a.partialB + r, a.mnPadB, a.kTiles);
+    // the two half partials' tile sums, each in tile order, then half 0 + half 1
+    const size_t hA = (size_t)a.kTiles * a.mnPadA, hB = (size_t)a.kTiles * a.mnPadB;
+    for (long long r = t; r < a.mnPadA; r += stride)
+        a.eA[r] = ordered_sum(a.partialA + r, a.mnPadA, a.kTiles) + ordered_sum(a.partialA + hA + r, a.mnPadA, a.kTiles);
+    for (long long r = t; r < a.mnPadB; r += stride)
+        a.eB[r] = ordered_sum(a.partialB + r, a.mnPadB, a.kTiles) + ordered_sum(a.partialB + hB + r, a.mnPadB, a.kTiles);
     for (long long w = t; w < a.bitmapWords; w += stride) a.bitmap[w] = 0ull;
     if (t == 0 && a.counter != nullptr) *a.counter = 0u;
 }

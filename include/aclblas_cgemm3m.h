@@ -192,7 +192,7 @@ aclblasStatus_t aclblasCgemm3mGroupedBatched(aclblasHandle_t handle, int count, 
  * correction when nonzero, else on for members with k >= 1024 as in aclblasCgemm3m; variant 5
  * puts every member on the one-launch kernel, variant 7 packs the persistent kernel's members
  * into planes with one launch first instead of reading their operands as stored: the same
- * bytes, kept for comparison). */
+ * bytes, kept for comparison; the standard identity always takes that packed form). */
 aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
                                                const aclblasOperation_t* transb, const int* m, const int* n,
                                                const int* k, const aclblasComplex* alpha,
@@ -201,7 +201,9 @@ aclblasStatus_t aclblasCgemm3mGroupedBatchedEx(aclblasHandle_t handle, int count
                                                const aclblasComplex* beta, aclblasComplex* const* C, const int* ldc,
                                                const aclblasCgemm3mConfig* config);
 
-/* Workspace bytes aclblasCgemm3mGroupedBatched needs for these problems. */
+/* Workspace bytes aclblasCgemm3mGroupedBatched needs for these problems: the larger of its two
+ * forms' needs (members read as stored, or packed into planes first when a leading dimension
+ * is too wide for the first form's 32-bit offsets; only the call sees the leading dimensions). */
 aclblasStatus_t aclblasCgemm3mGroupedBatchedGetWorkspaceSize(aclblasHandle_t handle, int count,
                                                              const aclblasOperation_t* transa,
                                                              const aclblasOperation_t* transb, const int* m,
