@@ -239,18 +239,29 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     // nothing is pinned, the fused plan has no K split and the 3 x (256 x 256 tiles) fill at
     // least half the CUs (8192^3 on MI355X, complex TFLOP/s fused vs separate: bf16x3 395 vs
     // 577, bf16x6 232 vs 319; 2048^3: 291 vs 323, 173 vs 198).
+    //
+    // BF16 products with too few 256 x 256 tiles for the CUs split k instead (tall-skinny and
+    // small outputs): about one workgroup per CU, each partition at least 256 deep, at most
+    // 128 partitions; merge_kernel sums each product's partitions in index order first.
     const bool pinned = cfg && (cfg->tile != 0 || cfg->kSplit != 0 || cfg->variant != 0);
     const long long tiles256 = ((m + 255) / 256) * ((n + 255) / 256);
+    long long splitParts = 1;
+    if (pl.format == FMT_BF16 && !pinned && 3 * tiles256 < cus / 2)
+        splitParts = std::max<long long>(
+            1, std::min<long long>({(cus + 3 * tiles256 - 1) / (3 * tiles256), 128, pl.kSteps * pl.bk / 256}));
     const bool products = (cfg && (cfg->variant == 4 || cfg->variant == 6)) ||
-                          (pl.format != FMT_F32 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2);
+                          (pl.format != FMT_F32 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2) ||
+                          splitParts > 1;
     if (products) {
-        if (cfg && (cfg->kSplit > 1 || (cfg->tile != 0 && cfg->tile != 128))) {
-            log_error(op, "the separate-products form has its own tile and no K split");
+        const bool pinnedSplit = cfg && cfg->variant == 4 && cfg->kSplit > 1 && pl.format == FMT_BF16;
+        if (cfg && ((cfg->kSplit > 1 && !pinnedSplit) || (cfg->tile != 0 && cfg->tile != 128))) {
+            log_error(op, "the separate-products form has its own tile; K split only for BF16 products in the "
+                          "register form");
             return false;
         }
         pl.path = 4;
         pl.bm = pl.bn = 256;
-        pl.parts = 1;
+        pl.parts = pinnedSplit ? std::min(cfg->kSplit, pl.kSteps) : (int)splitParts;
         pl.mPad = (int)round_up((size_t)m, 256);
         pl.nPad = (int)round_up((size_t)n, 256);
         // kernel: the register-prefetch form by default; variant 6 pins the LDS-DMA form (byte-
@@ -339,7 +350,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         if (!mul_ok((size_t)pl.mPad, (size_t)pl.nPad, mn)) return false;
         // K split: re and im planes per partition; separate products: one plane per product
         pl.partStride = pl.path == 4 ? mn : 2 * mn;
-        if (!mul_ok(pl.partStride * 4, (size_t)(pl.path == 4 ? 3 : pl.parts), bytes) ||
+        if (!mul_ok(pl.partStride * 4, (size_t)(pl.path == 4 ? 3 * pl.parts : pl.parts), bytes) ||
             !add_region(bytes, total, pl.offPartial))
             return false;
     }
@@ -550,6 +561,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
             mg.partStride = pl.partStride;
             mg.parts = pl.path == 4 ? 3 : pl.parts;
             mg.products = pl.path == 4 ? 1 : 0;
+            mg.splits = pl.path == 4 ? pl.parts : 1;
             mg.identity = pl.identity;
             mg.ldp = pl.mPad;
             mg.m = p.m;

@@ -31,8 +31,15 @@ __device__ __forceinline__ void merge_one(const MergeArgs& a, int i, int j)
     const size_t imOff = plane / 2;
     float re = 0.0f, im = 0.0f;
     if (a.products) {
-        // separate-products path: the combine of the fused tile, on the three product planes
-        combine3(a.identity, a.partial[o], a.partial[plane + o], a.partial[2 * plane + o], re, im);
+        // separate-products path: each product's K partitions summed in index order (starting
+        // from the first, so one partition is the plane itself), then the fused tile's combine
+        float pq[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            pq[q] = a.partial[(size_t)q * plane + o];
+            for (int sp = 1; sp < a.splits; ++sp) pq[q] += a.partial[(size_t)(3 * sp + q) * plane + o];
+        }
+        combine3(a.identity, pq[0], pq[1], pq[2], re, im);
     } else {
         for (int p = 0; p < a.parts; ++p) {
             re += a.partial[(size_t)p * plane + o];
@@ -68,11 +75,20 @@ __device__ __forceinline__ void merge_four(const MergeArgs& a, int i0, int j)
     const f4* P = reinterpret_cast<const f4*>(a.partial);
     f4 re = {0.0f, 0.0f, 0.0f, 0.0f}, im = {0.0f, 0.0f, 0.0f, 0.0f};
     if (a.products) {
-        const f4 p0 = P[o / 4], p1 = P[(plane + o) / 4], p2 = P[(2 * plane + o) / 4];
+        f4 pq[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            pq[q] = P[((size_t)q * plane + o) / 4];
+            for (int sp = 1; sp < a.splits; ++sp) {
+                const f4 v = P[((size_t)(3 * sp + q) * plane + o) / 4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pq[q][u] += v[u];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             float r, q;
-            combine3(a.identity, p0[u], p1[u], p2[u], r, q);
+            combine3(a.identity, pq[0][u], pq[1][u], pq[2][u], r, q);
             re[u] = r;
             im[u] = q;
         }
@@ -215,16 +231,12 @@ __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
     const float2* A = reinterpret_cast<const float2*>(a.A);
     const float2* B = reinterpret_cast<const float2*>(a.B);
     float2* C = reinterpret_cast<float2*>(a.C);
-    // Each wave scans 64 consecutive bitmap words at a time (one per lane) and handles the
-    // nonzero ones in increasing order.
-    for (long long w0 = wave * 64; w0 < a.words; w0 += waves * 64) {
-        const unsigned long long mine = w0 + lane < a.words ? a.bitmap[w0 + lane] : 0ull;
-        unsigned long long live = __ballot(mine != 0ull);
-        while (live != 0ull) {
-            const int src = __ffsll((long long)live) - 1;
-            live &= live - 1ull;
-            const long long w = w0 + src;
-            unsigned long long bits = __shfl(mine, src, 64);
+    // Each wave takes one bitmap word at a time (64 outputs; flagged outputs are rare, so
+    // spreading the words over many waves spreads them too) and handles its set bits in
+    // increasing order.
+    for (long long w = wave; w < a.words; w += waves) {
+        unsigned long long bits = a.bitmap[w];
+        {
             if (a.netlib) {
                 // one flagged output per lane, each in the reference BLAS's sequential FP32 order
                 if ((bits >> lane) & 1ull) {
@@ -243,19 +255,19 @@ __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
                 bits &= bits - 1ull;
                 const long long lin = w * 64 + b;
                 const long long i = lin % a.m, j = lin / a.m;
-                // lane sums k = lane + 64 t in increasing t; the loads of four t are in flight
-                // together, the adds stay in t order
+                // lane sums k = lane + 64 t in increasing t; the loads of sixteen t are in
+                // flight together (a long k is latency-bound otherwise), the adds stay in t order
                 double sr = 0.0, si = 0.0;
                 long long l = lane;
-                for (; l + 192 < a.k; l += 256) {
-                    double2 x[4], y[4];
+                for (; l + 64 * 15 < a.k; l += 64 * 16) {
+                    double2 x[16], y[16];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < 16; ++u) {
                         x[u] = load_op(A, a.lda, a.opA, i, l + 64 * u, false);   // op(A)(i, l)
                         y[u] = load_op(B, a.ldb, a.opB, l + 64 * u, j, false);   // op(B)(l, j)
                     }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
+                    for (int u = 0; u < 16; ++u) {
                         sr = fma(x[u].x, y[u].x, sr);
                         sr = fma(-x[u].y, y[u].y, sr);
                         si = fma(x[u].x, y[u].y, si);
@@ -302,8 +314,8 @@ __global__ __launch_bounds__(256) void recompute_grouped_kernel(const RecomputeA
 
 hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s)
 {
-    long long bx = (maxWords + 255) / 256;  // 4 waves per block, 64 bitmap words per wave
-    if (bx > 1024) bx = 1024;
+    long long bx = (maxWords + 3) / 4;  // 4 waves per block, a bitmap word per wave
+    if (bx > 4096) bx = 4096;
     if (bx < 1) bx = 1;
     if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
     hipLaunchKernelGGL(recompute_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items);
@@ -312,8 +324,8 @@ hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)
 {
-    long long blocks = (a.words + 255) / 256;  // 4 waves per block, 64 bitmap words per wave
-    if (blocks > 2048) blocks = 2048;
+    long long blocks = (a.words + 3) / 4;  // 4 waves per block, a bitmap word per wave
+    if (blocks > 16384) blocks = 16384;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
     return hipGetLastError();

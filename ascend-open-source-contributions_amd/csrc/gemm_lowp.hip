@@ -319,8 +319,11 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_kernel(GemmArgs g
     constexpr int NTERM = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
     constexpr int TB[6] = {NP == 1 ? 0 : 1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
     constexpr int TA[6] = {NP == 3 ? 1 : 0, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
-    int q, tmI, tnI;
-    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, q, tmI, tnI);
+    // plane = 3 sp + q: product q over K partition sp of g.parts (FP16: one partition)
+    int plane, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, plane, tmI, tnI);
+    const int q = plane % 3, sp = plane / 3;
+    const int ks0 = (int)((long long)g.kSteps * sp / g.parts), ks1 = (int)((long long)g.kSteps * (sp + 1) / g.parts);
     if (F16S && f16_tile_mixed(g, q, tmI, tnI) != RESCALE) return;
     for (int pass = 0; pass < PASSES; ++pass) {
         const int lane = threadIdx.x & 63;
@@ -385,28 +388,29 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_kernel(GemmArgs g
 
         // Ring of NSET register sets: sets 0..NSET-2 preloaded; stage st computes from set
         // st % NSET after issuing the loads of stage st + NSET - 1 into the set freed last.
-        const int nst = g.kSteps;
+        // Stages ks0 .. ks1-1 of this K partition (absolute stage indices).
+        const int nst = ks1 - ks0;
         Set S[NSET];
 #pragma unroll
-        for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nst - 1));
+        for (int u = 0; u + 1 < NSET; ++u) load(S[u], ks0 + min(u, nst - 1));
         int st = 0;
         for (; st + NSET <= nst; st += NSET) {
 #pragma unroll
             for (int u = 0; u < NSET; ++u) {
                 // Unconditional (clamped to the last stage): a load skipped on one path would make
                 // the wait before the next MFMAs cover the loads just issued.
-                load(S[(u + NSET - 1) % NSET], min(st + u + NSET - 1, nst - 1));
+                load(S[(u + NSET - 1) % NSET], ks0 + min(st + u + NSET - 1, nst - 1));
                 // Keep the loads ahead of the MFMAs: the scheduler would otherwise sink each load
                 // next to its first use and wait on it there.
                 __builtin_amdgcn_sched_barrier(0);
-                mfma(S[u], st + u);
+                mfma(S[u], ks0 + st + u);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
 #pragma unroll
         for (int u = 0; u + 1 < NSET; ++u)
-            if (st + u < nst) mfma(S[u], st + u);
-        store_products<F16S, TN>(g, acc, q, tmI, tnI, wm, wn, pass * TN, lane, eCur);
+            if (st + u < nst) mfma(S[u], ks0 + st + u);
+        store_products<F16S, TN>(g, acc, plane, tmI, tnI, wm, wn, pass * TN, lane, eCur);
     }
 }
 
@@ -521,7 +525,8 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmAr
 }
 
 // g.tilesM / g.tilesN count 256 x 256 tiles (mPad, nPad multiples of 256); g.partial holds
-// three FP32 planes of mPad x nPad, g.partStride floats apart; the whole k range (no split).
+// 3 g.parts FP32 planes of mPad x nPad, g.partStride floats apart: plane 3 s + q is product q
+// over the s-th of g.parts K partitions (stages kSteps s / parts .. kSteps (s+1) / parts).
 // pieces: 2 bf16x3, 3 bf16x6, 1 FP16 block-scaled. The LDS form wants kSteps in whole groups
 // of lowp_products_lds_unroll(pieces) stages.
 int lowp_products_lds_unroll(int pieces)
@@ -533,7 +538,10 @@ int lowp_products_lds_unroll(int pieces)
 
 hipError_t launch_gemm_lowp_products(const GemmArgs& g, int pieces, bool lds, hipStream_t s)
 {
-    const int blocks = 3 * g.tilesM * g.tilesN;
+    // K partitions: the register form of the BF16 products only (FP16's block exponents and
+    // the LDS form's unrolled stage groups assume the whole k range)
+    if (g.parts < 1 || (g.parts > 1 && (lds || pieces == 1))) return hipErrorInvalidValue;
+    const int blocks = 3 * g.parts * g.tilesM * g.tilesN;
     if (lds) {
         if (g.kSteps % lowp_products_lds_unroll(pieces)) return hipErrorInvalidValue;
         if (pieces == 2)

@@ -517,7 +517,8 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                     energies.push_back(EnergyArgs{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles,
                                                   const_cast<double*>(g.eA), const_cast<double*>(g.eB), g.bitmap,
                                                   pl.bitmapWords, nullptr});
-                    maxEnergyWork = std::max<long long>({maxEnergyWork, pl.mPad, pl.nPad, pl.bitmapWords});
+                    maxEnergyWork =
+                        std::max<long long>(maxEnergyWork, energy_work(pl.mPad, pl.nPad, pl.kTiles, pl.bitmapWords));
                 } else {
                     energies.push_back(EnergyArgs{nullptr, 0, nullptr, 0, 0, nullptr, nullptr, g.bitmap,
                                                   pl.bitmapWords, energies.empty() ? queue0 : nullptr});
@@ -526,7 +527,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                         sums.push_back(EnergyArgs{pa.energyPartial, pl.mPad, pb.energyPartial, pl.nPad, pl.kTiles,
                                                   const_cast<double*>(g.eA), const_cast<double*>(g.eB), nullptr, 0,
                                                   nullptr});
-                        maxSumWork = std::max<long long>({maxSumWork, pl.mPad, pl.nPad});
+                        maxSumWork = std::max<long long>(maxSumWork, energy_work(pl.mPad, pl.nPad, pl.kTiles, 0));
                     }
                 }
                 RecomputeArgs r{};

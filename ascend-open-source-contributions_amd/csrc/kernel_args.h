@@ -140,14 +140,14 @@ struct SmallArgs {      // one-launch path for small problems: complex inputs re
 
 struct MergeArgs {      // K-split epilogue: fixed-order partition sum + alpha/beta
     const float* partial; size_t partStride; int parts; int ldp;
-    int products;       // 1: partial holds the three real products P0, P1, P2 (one plane each)
+    int products;       // 1: partial holds the three real products P0, P1, P2 (one plane each),
+                        // per K partition: plane 3 s + q for partition s of `splits`
     int identity;       // Identity used to combine P0..P2 when products == 1
+    int splits;         // products == 1: K partitions of each product (>= 1), summed in index order
     int m, n;
     float alphaR, alphaI, betaR, betaI; int betaZero;
     void* C; long long ldc;
     int correction;
-    int energyInline;                       // grouped FP32 kernel, reading the operands as stored:
-                                            // the energies are formed from its loads (eA, eB unused)
     const double* eA; const double* eB; double corrScale, corrTau;
     unsigned long long* bitmap;
     SourceOperands src; // non-finite outputs are recomputed from these

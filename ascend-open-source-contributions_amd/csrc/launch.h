@@ -30,6 +30,8 @@ hipError_t launch_gemm_f32_grouped_quads(const GroupProblem* probs, const GroupT
                                          unsigned int* queue, int blocks, bool raw, hipStream_t s);
 hipError_t launch_merge_grouped(const GroupMerge* items, int nitems, int maxOutputs, hipStream_t s);
 hipError_t launch_energy_grouped(const EnergyArgs* items, int nitems, long long maxWork, hipStream_t s);
+// threads an energy item wants (maxWork of launch_energy_grouped is the maximum over its items)
+long long energy_work(int mnPadA, int mnPadB, int kTiles, long long bitmapWords);
 hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s);
 
 }  // namespace c3m

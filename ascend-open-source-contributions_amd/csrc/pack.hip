@@ -395,21 +395,60 @@ hipError_t launch_scale_summary(const ScaleSummaryArgs& a, hipStream_t s)
 }
 
 // Row energies e[r] = (sum over k tiles, in index order, of the pack kernel's half-0 partials)
-// + (the same for half 1), the half-sum order of energy_acc; also
-// clears the correction bitmap and the flagged-output counter for this call.
-// sum over kt of p[kt * ld] in index order; eight loads issued ahead of their adds
-__device__ __forceinline__ double ordered_sum(const double* p, size_t ld, int n)
+// + (the same for half 1), the half-sum order of energy_acc; also clears the correction
+// bitmap and the flagged-output counter for this call. Each half is one sequential chain of
+// kTiles additions (the order is fixed, so it cannot be split into a tree). Up to 256 tiles a
+// thread runs a row's two chains side by side, 16 loads in flight per batch; longer chains
+// (tall-skinny problems: 1024 tiles at k = 65536) would be latency-bound that way, so a
+// wave takes the row: every lane loads up to 16 tiles of a block of 1024 at once, then the
+// wave adds them in tile order, lane by lane (readlane), the running sum the same in every
+// lane. Tiles past kTiles load as zero, which adds nothing (the sums are >= 0).
+constexpr int kLongChain = 256;
+
+__device__ __forceinline__ double ordered_sum2(const double* p0, const double* p1, size_t ld, int n)
 {
-    double s = 0.0;
+    double s0 = 0.0, s1 = 0.0;
     int kt = 0;
     for (; kt + 8 <= n; kt += 8) {
-        double v[8];
+        double v[8], w[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = p[(size_t)(kt + u) * ld];
+        for (int u = 0; u < 8; ++u) {
+            v[u] = p0[(size_t)(kt + u) * ld];
+            w[u] = p1[(size_t)(kt + u) * ld];
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
+        for (int u = 0; u < 8; ++u) {
+            s0 += v[u];
+            s1 += w[u];
+        }
     }
-    for (; kt < n; ++kt) s += p[(size_t)kt * ld];
+    for (; kt < n; ++kt) {
+        s0 += p0[(size_t)kt * ld];
+        s1 += p1[(size_t)kt * ld];
+    }
+    return s0 + s1;
+}
+
+__device__ __forceinline__ double wave_ordered_sum(const double* p, size_t ld, int n, int lane)
+{
+    double s = 0.0;
+    for (int t0 = 0; t0 < n; t0 += 1024) {
+        double v[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int t = t0 + lane * 16 + c;
+            v[c] = t < n ? p[(size_t)t * ld] : 0.0;
+        }
+        const int live = min(64, (n - t0 + 15) / 16);  // lanes holding tiles of this block
+        for (int l = 0; l < live; ++l) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int lo = __builtin_amdgcn_readlane(__double2loint(v[c]), l);
+                const int hi = __builtin_amdgcn_readlane(__double2hiint(v[c]), l);
+                s += __hiloint2double(hi, lo);
+            }
+        }
+    }
     return s;
 }
 
@@ -419,10 +458,24 @@ __device__ __forceinline__ void energy_body(const EnergyArgs& a)
     const long long stride = (long long)gridDim.x * 256;
     // the two half partials' tile sums, each in tile order, then half 0 + half 1
     const size_t hA = (size_t)a.kTiles * a.mnPadA, hB = (size_t)a.kTiles * a.mnPadB;
-    for (long long r = t; r < a.mnPadA; r += stride)
-        a.eA[r] = ordered_sum(a.partialA + r, a.mnPadA, a.kTiles) + ordered_sum(a.partialA + hA + r, a.mnPadA, a.kTiles);
-    for (long long r = t; r < a.mnPadB; r += stride)
-        a.eB[r] = ordered_sum(a.partialB + r, a.mnPadB, a.kTiles) + ordered_sum(a.partialB + hB + r, a.mnPadB, a.kTiles);
+    if (a.kTiles <= kLongChain) {
+        for (long long r = t; r < a.mnPadA; r += stride)
+            a.eA[r] = ordered_sum2(a.partialA + r, a.partialA + hA + r, a.mnPadA, a.kTiles);
+        for (long long r = t; r < a.mnPadB; r += stride)
+            a.eB[r] = ordered_sum2(a.partialB + r, a.partialB + hB + r, a.mnPadB, a.kTiles);
+    } else {
+        const int lane = threadIdx.x & 63;
+        const long long rows = (long long)a.mnPadA + a.mnPadB;
+        for (long long r = t >> 6; r < rows; r += stride >> 6) {
+            const bool isA = r < a.mnPadA;
+            const long long rr = isA ? r : r - a.mnPadA;
+            const size_t ld = isA ? a.mnPadA : a.mnPadB;
+            const double* p = (isA ? a.partialA : a.partialB) + rr;
+            const double e = wave_ordered_sum(p, ld, a.kTiles, lane) +
+                             wave_ordered_sum(p + (isA ? hA : hB), ld, a.kTiles, lane);
+            if (lane == 0) (isA ? a.eA : a.eB)[rr] = e;
+        }
+    }
     for (long long w = t; w < a.bitmapWords; w += stride) a.bitmap[w] = 0ull;
     if (t == 0 && a.counter != nullptr) *a.counter = 0u;
 }
@@ -446,10 +499,16 @@ hipError_t launch_energy_grouped(const EnergyArgs* items, int nitems, long long 
     return hipGetLastError();
 }
 
+long long energy_work(int mnPadA, int mnPadB, int kTiles, long long bitmapWords)
+{
+    long long work = mnPadA > mnPadB ? mnPadA : mnPadB;  // a thread per row
+    if (kTiles > kLongChain) work = 64LL * ((long long)mnPadA + mnPadB);  // a wave per row
+    return bitmapWords > work ? bitmapWords : work;
+}
+
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s)
 {
-    long long work = a.mnPadA > a.mnPadB ? a.mnPadA : a.mnPadB;
-    if (a.bitmapWords > work) work = a.bitmapWords;
+    const long long work = energy_work(a.mnPadA, a.mnPadB, a.kTiles, a.bitmapWords);
     long long blocks = (work + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;

@@ -125,7 +125,8 @@ def test_config_validation(h):
                 cgemm3m.config("bf16x3", variant=5), cgemm3m.config(variant=5, k_split=17),
                 cgemm3m.config(variant=5, identity=3), cgemm3m.config("fp16", variant=3),
                 cgemm3m.config(tile=16), cgemm3m.config(tile=16, variant=3),
-                cgemm3m.config("bf16x3", variant=4, k_split=2), cgemm3m.config("bf16x3", variant=4, tile=64),
+                cgemm3m.config("fp16", variant=4, k_split=2), cgemm3m.config("bf16x3", variant=6, k_split=2),
+                    cgemm3m.config("bf16x3", variant=4, tile=64),
                 cgemm3m.config("bf16x3", variant=3), cgemm3m.config(tile=64, variant=3)):
         st = cgemm3m.cgemm3m(h, "N", "N", 16, 16, 16, 1, FAKE, 16, FAKE, 16, 0, FAKE, 16, cfg, check=False)
         assert st == _lib.INVALID_VALUE

@@ -149,6 +149,37 @@ def test_tall_skinny_ksplit(gpu, parts):
         check_against_oracle(out, d["gold"], d["C"], m, n, m, "tall-skinny-auto")
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16x6"])
+def test_tall_skinny_bf16_products_ksplit(gpu, prec):
+    """256 x 256 x 65536 on the BF16 forms: the planner splits k over the separate-products
+    kernel (too few 256 x 256 tiles for the CUs), and merge_kernel sums each product's
+    partitions in index order. FP64 error no worse than twice Netlib's own (normwise; the
+    bf16x3 products drop the lo*lo term, so its mean elementwise error is about 3x the FP32
+    path's and is bounded at 4x Netlib's), byte-identical reruns."""
+    import cgemm3m
+
+    m = n = 256
+    k = 65536
+    d = _tall_skinny_inputs(gpu)
+    cfg = cgemm3m.config(prec)
+    print("case tall-skinny", prec)
+    sys.stdout.flush()
+    st, out = run_gpu(gpu, "N", "N", m, n, k, ALPHA2, d["A"], m, d["B"], k, BETA2, d["C"], m, cfg, d["dA"], d["dB"])
+    assert st == 0
+    plan = gpu.last_plan(sync=True)
+    print("plan", plan)
+    assert plan["path"] == 4 and plan["parts"] > 1
+    dOut = to_dev(out)
+    mine = cgemm3m.residual(gpu, "N", "N", m, n, k, ALPHA2, ptr(d["dA"]), m, ptr(d["dB"]), k, BETA2, ptr(d["dC0"]),
+                            ptr(dOut), m)
+    ref = d["netlib_err"]
+    print("fp64 error: ours %s netlib %s" % (mine, ref))
+    assert mine["normwise"] <= 2 * ref["normwise"] and mine["mean_rel"] <= 4 * ref["mean_rel"]
+    st2, out2 = run_gpu(gpu, "N", "N", m, n, k, ALPHA2, d["A"], m, d["B"], k, BETA2, d["C"], m, cfg, d["dA"],
+                        d["dB"])
+    assert out2.tobytes() == out.tobytes(), "second run differs"
+
+
 @pytest.mark.parametrize("identity", [1, 2])
 @pytest.mark.parametrize("tile", [64, 128])
 def test_identity_and_tile(gpu, identity, tile):
@@ -238,6 +269,9 @@ INT_CASES = [
     ("fp16", 300, 260, 512, "TC", dict(precision="fp16")),
     ("bf16x3-products", 2048, 1024, 512, "NN", dict(precision="bf16x3", variant=4)),
     ("fp16-products", 2048, 1024, 512, "CN", dict(precision="fp16", variant=4)),
+    ("bf16x3-products-ksplit5", 300, 260, 2048, "CN", dict(precision="bf16x3", variant=4, k_split=5)),
+    ("bf16x6-products-ksplit3", 260, 300, 1024, "TC", dict(precision="bf16x6", variant=4, k_split=3)),
+    ("bf16x3-tall-skinny-auto", 200, 256, 8192, "NT", dict(precision="bf16x3")),
 ]
 
 
