@@ -198,7 +198,9 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     pl.kSteps = pl.kPad / pl.bk;
     const long long tiles = ((m + pl.bm - 1) / pl.bm) * ((n + pl.bn - 1) / pl.bn);
     if (cfg && cfg->kSplit != 0) {
-        if (cfg->kSplit < 1 || cfg->kSplit > 16) {
+        // up to 128 partitions for the BF16 separate products (variants 4 and 7), 16 elsewhere
+        const int maxSplit = (pl.format == FMT_BF16 && (cfg->variant == 4 || cfg->variant == 7)) ? 128 : 16;
+        if (cfg->kSplit < 1 || cfg->kSplit > maxSplit) {
             log_error(op, "invalid kSplit %d", cfg->kSplit);
             return false;
         }
@@ -225,7 +227,8 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         return false;
     }
     pl.path = pl.parts > 1 ? 2 : 1;
-    if (cfg && (cfg->variant < 0 || cfg->variant > 6 ||
+    if (cfg && (cfg->variant < 0 || cfg->variant > 7 ||
+                (cfg->variant == 7 && pl.format != FMT_BF16) ||
                 (cfg->variant == 6 && pl.format == FMT_F32) ||
                 (cfg->variant == 5 && (pl.format != FMT_F32 || pl.identity == ID_FOUR)) ||
                 (cfg->variant == 3 && (pl.format != FMT_F32 || (pl.bm != 128 && pl.identity != ID_FOUR))) ||
@@ -249,25 +252,42 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     if (pl.format == FMT_BF16 && !pinned && 3 * tiles256 < cus / 2)
         splitParts = std::max<long long>(
             1, std::min<long long>({(cus + 3 * tiles256 - 1) / (3 * tiles256), 128, pl.kSteps * pl.bk / 256}));
+    //
+    // With few outputs (at most 16 tiles of 128 x 128) the BF16 products read the operands as
+    // stored instead (variant 7 pins it): no pack launch, the plane values and BF16 pieces
+    // formed in registers, the same bits. About one workgroup per CU over the K partitions.
+    // Every lane offset of that kernel must fit in 31 bits (72 leading dimensions of 8 bytes).
+    const long long tiles128 = ((m + 127) / 128) * ((n + 127) / 128);
+    const bool rawFits = (long long)std::max(p.lda, p.ldb) * 8 * 72 < (1LL << 31);
+    if (cfg && cfg->variant == 7 && !rawFits) {
+        log_error(op, "a leading dimension of %d is too wide for variant 7", std::max(p.lda, p.ldb));
+        return false;
+    }
+    const bool raw = (cfg && cfg->variant == 7) || (splitParts > 1 && rawFits && tiles128 <= 16);
     const bool products = (cfg && (cfg->variant == 4 || cfg->variant == 6)) ||
                           (pl.format != FMT_F32 && !pinned && pl.parts == 1 && 3 * tiles256 >= cus / 2) ||
-                          splitParts > 1;
+                          splitParts > 1 || raw;
     if (products) {
-        const bool pinnedSplit = cfg && cfg->variant == 4 && cfg->kSplit > 1 && pl.format == FMT_BF16;
+        const bool pinnedSplit =
+            cfg && (cfg->variant == 4 || cfg->variant == 7) && cfg->kSplit > 1 && pl.format == FMT_BF16;
         if (cfg && ((cfg->kSplit > 1 && !pinnedSplit) || (cfg->tile != 0 && cfg->tile != 128))) {
             log_error(op, "the separate-products form has its own tile; K split only for BF16 products in the "
                           "register form");
             return false;
         }
         pl.path = 4;
-        pl.bm = pl.bn = 256;
+        pl.raw = raw;
+        pl.bm = pl.bn = raw ? 128 : 256;
+        if (raw && !pinnedSplit)
+            splitParts = std::max<long long>(
+                1, std::min<long long>({(cus + tiles128 - 1) / tiles128, 128, pl.kSteps * pl.bk / 256}));
         pl.parts = pinnedSplit ? std::min(cfg->kSplit, pl.kSteps) : (int)splitParts;
-        pl.mPad = (int)round_up((size_t)m, 256);
-        pl.nPad = (int)round_up((size_t)n, 256);
+        pl.mPad = (int)round_up((size_t)m, raw ? 128 : 256);
+        pl.nPad = (int)round_up((size_t)n, raw ? 128 : 256);
         // kernel: the register-prefetch form by default; variant 6 pins the LDS-DMA form (byte-
         // identical, measured slower: bf16x3 at 8192^3 7.52 ms against 6.94 ms), whose unrolled
         // stage loop wants k in whole groups of its unroll depth (16-deep stages)
-        pl.variant = (cfg && cfg->variant == 6) ? 5 : 3;
+        pl.variant = raw ? 6 : ((cfg && cfg->variant == 6) ? 5 : 3);
         if (pl.variant == 5) {
             pl.kPad = (int)round_up((size_t)k, 16 * (size_t)lowp_products_lds_unroll(pl.pieces));
             pl.kSteps = pl.kPad / pl.bk;
@@ -342,6 +362,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     if (!mul_ok((size_t)pl.nPad, (size_t)pl.kPad, t2) || !mul_ok(t2, elem, pl.bPlaneBytes)) return false;
     pl.planesA = pl.planesB = 3 * pl.pieces;
     if (pl.identity == ID_FOUR) pl.planesB = 2;  // (Br, Bi)
+    if (pl.raw) pl.aPlaneBytes = pl.bPlaneBytes = 0;  // no planes: the products read A and B as stored
     size_t total = 0, bytes = 0;
     if (!mul_ok(pl.aPlaneBytes, (size_t)pl.planesA, bytes) || !add_region(bytes, total, pl.offA)) return false;
     if (!mul_ok(pl.bPlaneBytes, (size_t)pl.planesB, bytes) || !add_region(bytes, total, pl.offB)) return false;
@@ -463,7 +484,7 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
     } else {
         const PackArgs pa = pack_args(p, pl, ws, 0);
         const PackArgs pb = pack_args(p, pl, ws, 1);
-        if (pl.format == FMT_F16) {
+        if (pl.format == FMT_F16 && !pl.raw) {
             StageTimer t(th, ST_PACK);
             auto scale_args = [](const PackArgs& x) {
                 return ScaleArgs{x.src, x.ld, x.mn, x.kk, x.mnPad, x.kPad, x.mnContig, x.conj, x.role,
@@ -476,9 +497,12 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
                                 reinterpret_cast<int*>(ws + pl.offSumB)};
             if (e == hipSuccess) e = launch_scale_summary(sm, s);
         }
-        if (e == hipSuccess) {
+        if (e == hipSuccess && !pl.raw) {
             StageTimer t(th, ST_PACK);
             e = launch_pack_pair(pa, pb, s);
+        } else if (e == hipSuccess && pl.correction) {  // products read as stored: energies only
+            StageTimer t(th, ST_PACK);
+            e = launch_energy_partial_pair(pa, pb, s);
         }
         double* eA = nullptr;
         double* eB = nullptr;
@@ -526,8 +550,8 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.ldc = p.ldc;
         g.partial = (pl.parts > 1 || pl.path == 4) ? reinterpret_cast<float*>(ws + pl.offPartial) : nullptr;
         if (pl.path == 4) {
-            g.tilesM = pl.mPad / 256;
-            g.tilesN = pl.nPad / 256;
+            g.tilesM = pl.mPad / pl.bm;
+            g.tilesN = pl.nPad / pl.bn;
         }
         g.partStride = pl.partStride;
         g.scaleA = pa.scaleExp;
@@ -546,7 +570,9 @@ aclblasStatus_t enqueue(aclblasHandle_t h, const Problem& p, const Plan& pl, cha
         g.src = SourceOperands{p.A, p.lda, (int)p.ta, p.B, p.ldb, (int)p.tb, p.k};
         if (e == hipSuccess) {
             StageTimer t(th, ST_GEMM);
-            if (pl.path == 4)
+            if (pl.path == 4 && pl.raw)
+                e = launch_gemm_bf16_raw_products(g, pl.pieces, s);
+            else if (pl.path == 4)
                 e = launch_gemm_lowp_products(g, pl.pieces, pl.variant == 5, s);
             else if (pl.format == FMT_F32)
                 e = launch_gemm_f32(g, pl.bm, pl.bn, pl.variant, s);

@@ -10,17 +10,35 @@ namespace c3m {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// 16-byte lane vector as an ext-vector type: arrays of it stay in VGPRs (an array of the
+// HIP struct uint4 can be moved to LDS by the compiler, serialising the prefetch loads).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-// Round-to-nearest-even FP32 -> BF16 bits (NaN stays a quiet NaN).
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x)
+// Eight FP32 values split into NP BF16 pieces, each packed as one MFMA operand (element s in
+// bits 16 (s % 2) of word s / 2): piece p = bf16(what pieces 0..p-1 left), rounded to nearest
+// even by the hardware conversion (v_cvt_pk_bf16_f32; FP32 denormals are kept, as the
+// kernels run with denormals on); the remainder x - float(piece) is exact. The pack kernel
+// and the kernels reading the operands as stored both split with this, so their planes agree
+// bit for bit (NaN payloads aside: a NaN output is recomputed from the stored operands).
+template <int NP>
+__device__ __forceinline__ void bf16_split8(float (&x)[8], u32x4 (&out)[NP])
 {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const bf16x2 h = {(__bf16)x[2 * w], (__bf16)x[2 * w + 1]};
+            const unsigned u = __builtin_bit_cast(unsigned, h);
+            out[p][w] = u;
+            if (p + 1 < NP) {
+                x[2 * w] -= __uint_as_float(u << 16);
+                x[2 * w + 1] -= __uint_as_float(u & 0xffff0000u);
+            }
+        }
+    }
 }
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Logical output tile of hardware block `bid`. Workgroups are dealt round-robin to the 8
 // XCDs, each with its own L2; remapping gives every XCD one contiguous range of logical

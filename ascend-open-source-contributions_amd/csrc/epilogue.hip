@@ -10,10 +10,10 @@
 //   recompute_kernel rounding correction (reference K5-K8, cgemm3m_rounding_kernel.cpp:
 //                    132-497, re-designed): every output flagged by the GEMM epilogue (which
 //                    left C's input in place there) is recomputed from the original
-//                    complex64 inputs in FP64, each lane
-//                    taking a fixed residue class of k in increasing order and the 64 lane
-//                    sums combined by a fixed butterfly, then alpha/beta applied in FP64
-//                    and the result rounded once to FP32.
+//                    complex64 inputs in FP64 by one workgroup, each thread taking a fixed
+//                    residue class of k mod 256 in increasing order, each wave's 64 sums
+//                    combined by a fixed butterfly and the four wave sums in wave order, then
+//                    alpha/beta applied in FP64 and the result rounded once to FP32.
 //   residual_kernel  FP64 complex GEMM of the same inputs compared with the FP32 result
 //                    (error report for tests and benches only).
 #include "device_util.h"
@@ -225,80 +225,88 @@ __device__ __forceinline__ double2 load_op(const float2* X, long long ld, int op
 
 __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
 {
-    const int lane = threadIdx.x & 63;
-    const long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
-    const long long waves = ((long long)gridDim.x * 256) >> 6;
+    __shared__ double part[4][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float2* A = reinterpret_cast<const float2*>(a.A);
     const float2* B = reinterpret_cast<const float2*>(a.B);
     float2* C = reinterpret_cast<float2*>(a.C);
-    // Each wave takes one bitmap word at a time (64 outputs; flagged outputs are rare, so
-    // spreading the words over many waves spreads them too) and handles its set bits in
-    // increasing order.
-    for (long long w = wave; w < a.words; w += waves) {
-        unsigned long long bits = a.bitmap[w];
-        {
-            if (a.netlib) {
-                // one flagged output per lane, each in the reference BLAS's sequential FP32 order
-                if ((bits >> lane) & 1ull) {
-                    const long long lin = w * 64 + lane;
-                    const long long i = lin % a.m, j = lin / a.m;
-                    const float2 c0 = a.betaZero ? make_float2(0.0f, 0.0f) : C[(size_t)j * a.ldc + i];
-                    C[(size_t)j * a.ldc + i] = netlib_output(A, a.lda, a.opA, B, a.ldb, a.opB, i, j, a.k,
-                                                             make_float2(a.alphaRf, a.alphaIf),
-                                                             make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
-                    if (a.counter != nullptr) atomicAdd(a.counter, 1u);
-                }
-                continue;
-            }
-            while (bits != 0ull) {
-                const int b = __ffsll((long long)bits) - 1;
-                bits &= bits - 1ull;
-                const long long lin = w * 64 + b;
+    // Each workgroup takes one bitmap word at a time (64 outputs; flagged outputs are rare, so
+    // spreading the words over many workgroups spreads them too) and handles its set bits in
+    // increasing order, all 256 threads on one output.
+    for (long long w = blockIdx.x; w < a.words; w += gridDim.x) {
+        const unsigned long long word = a.bitmap[w];
+        unsigned long long bits = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(word >> 32)) << 32) |
+                                  (unsigned)__builtin_amdgcn_readfirstlane((int)word);
+        if (a.netlib) {
+            // one flagged output per thread, each in the reference BLAS's sequential FP32 order
+            if (tid < 64 && ((bits >> tid) & 1ull)) {
+                const long long lin = w * 64 + tid;
                 const long long i = lin % a.m, j = lin / a.m;
-                // lane sums k = lane + 64 t in increasing t; the loads of sixteen t are in
-                // flight together (a long k is latency-bound otherwise), the adds stay in t order
-                double sr = 0.0, si = 0.0;
-                long long l = lane;
-                for (; l + 64 * 15 < a.k; l += 64 * 16) {
-                    double2 x[16], y[16];
+                const float2 c0 = a.betaZero ? make_float2(0.0f, 0.0f) : C[(size_t)j * a.ldc + i];
+                C[(size_t)j * a.ldc + i] = netlib_output(A, a.lda, a.opA, B, a.ldb, a.opB, i, j, a.k,
+                                                         make_float2(a.alphaRf, a.alphaIf),
+                                                         make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
+                if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+            }
+            continue;
+        }
+        while (bits != 0ull) {
+            const int b = __ffsll((long long)bits) - 1;
+            bits &= bits - 1ull;
+            const long long lin = w * 64 + b;
+            const long long i = lin % a.m, j = lin / a.m;
+            // thread t sums k = t + 256 u in increasing u (the loads of sixteen u in flight
+            // together, the adds in u order); each wave's 64 sums by a fixed butterfly; the
+            // four wave sums added in wave order
+            double sr = 0.0, si = 0.0;
+            long long l = tid;
+            for (; l + 256 * 15 < a.k; l += 256 * 16) {
+                double2 x[16], y[16];
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        x[u] = load_op(A, a.lda, a.opA, i, l + 64 * u, false);   // op(A)(i, l)
-                        y[u] = load_op(B, a.ldb, a.opB, l + 64 * u, j, false);   // op(B)(l, j)
-                    }
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        sr = fma(x[u].x, y[u].x, sr);
-                        sr = fma(-x[u].y, y[u].y, sr);
-                        si = fma(x[u].x, y[u].y, si);
-                        si = fma(x[u].y, y[u].x, si);
-                    }
-                }
-                for (; l < a.k; l += 64) {
-                    const double2 x = load_op(A, a.lda, a.opA, i, l, false);
-                    const double2 y = load_op(B, a.ldb, a.opB, l, j, false);
-                    sr = fma(x.x, y.x, sr);
-                    sr = fma(-x.y, y.y, sr);
-                    si = fma(x.x, y.y, si);
-                    si = fma(x.y, y.x, si);
+                for (int u = 0; u < 16; ++u) {
+                    x[u] = load_op(A, a.lda, a.opA, i, l + 256 * u, false);   // op(A)(i, l)
+                    y[u] = load_op(B, a.ldb, a.opB, l + 256 * u, j, false);   // op(B)(l, j)
                 }
 #pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) {
-                    sr += __shfl_xor(sr, d, 64);
-                    si += __shfl_xor(si, d, 64);
-                }
-                if (lane == 0) {
-                    double cr = a.alphaR * sr - a.alphaI * si;
-                    double ci = a.alphaR * si + a.alphaI * sr;
-                    if (!a.betaZero) {
-                        const float2 c0 = C[(size_t)j * a.ldc + i];  // the epilogue left the input there
-                        cr += a.betaR * c0.x - a.betaI * c0.y;
-                        ci += a.betaR * c0.y + a.betaI * c0.x;
-                    }
-                    C[(size_t)j * a.ldc + i] = make_float2((float)cr, (float)ci);
-                    if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+                for (int u = 0; u < 16; ++u) {
+                    sr = fma(x[u].x, y[u].x, sr);
+                    sr = fma(-x[u].y, y[u].y, sr);
+                    si = fma(x[u].x, y[u].y, si);
+                    si = fma(x[u].y, y[u].x, si);
                 }
             }
+            for (; l < a.k; l += 256) {
+                const double2 x = load_op(A, a.lda, a.opA, i, l, false);
+                const double2 y = load_op(B, a.ldb, a.opB, l, j, false);
+                sr = fma(x.x, y.x, sr);
+                sr = fma(-x.y, y.y, sr);
+                si = fma(x.x, y.y, si);
+                si = fma(x.y, y.x, si);
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                sr += __shfl_xor(sr, d, 64);
+                si += __shfl_xor(si, d, 64);
+            }
+            if (lane == 0) {
+                part[wave][0] = sr;
+                part[wave][1] = si;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                sr = ((part[0][0] + part[1][0]) + part[2][0]) + part[3][0];
+                si = ((part[0][1] + part[1][1]) + part[2][1]) + part[3][1];
+                double cr = a.alphaR * sr - a.alphaI * si;
+                double ci = a.alphaR * si + a.alphaI * sr;
+                if (!a.betaZero) {
+                    const float2 c0 = C[(size_t)j * a.ldc + i];  // the epilogue left the input there
+                    cr += a.betaR * c0.x - a.betaI * c0.y;
+                    ci += a.betaR * c0.y + a.betaI * c0.x;
+                }
+                C[(size_t)j * a.ldc + i] = make_float2((float)cr, (float)ci);
+                if (a.counter != nullptr) atomicAdd(a.counter, 1u);
+            }
+            __syncthreads();  // part[] is rewritten by the next output
         }
     }
 }
@@ -314,7 +322,7 @@ __global__ __launch_bounds__(256) void recompute_grouped_kernel(const RecomputeA
 
 hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s)
 {
-    long long bx = (maxWords + 3) / 4;  // 4 waves per block, a bitmap word per wave
+    long long bx = maxWords;  // a bitmap word per workgroup
     if (bx > 4096) bx = 4096;
     if (bx < 1) bx = 1;
     if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
@@ -324,7 +332,7 @@ hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)
 {
-    long long blocks = (a.words + 3) / 4;  // 4 waves per block, a bitmap word per wave
+    long long blocks = a.words;  // a bitmap word per workgroup
     if (blocks > 16384) blocks = 16384;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);

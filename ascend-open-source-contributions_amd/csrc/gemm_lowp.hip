@@ -18,14 +18,12 @@
 //    (kScaleNone) keep the running factor. Exact while the running sum stays in FP32 range:
 //    a rescale can overflow only if two blocks along k differ by about 2^80 in magnitude,
 //    and a non-finite output is flagged by the correction screen.
+#include <type_traits>
 #include "device_util.h"
 #include "launch.h"
 
 namespace c3m {
 
-// 16-byte lane vector as an ext-vector type: arrays of it stay in VGPRs (an array of the
-// HIP struct uint4 can be moved to LDS by the compiler, serialising the prefetch loads).
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // FP16 separate-products kernels: register sets in flight (register form) and LDS ring depth
 #ifndef C3M_F16_NSET
@@ -522,6 +520,260 @@ __global__ __launch_bounds__(256, 1) void gemm3m_lowp_products_lds_kernel(GemmAr
     // no load may still be writing LDS when the workgroup ends
     __builtin_amdgcn_s_waitcnt(0);
     store_products<F16S, TN>(g, acc, q, tmI, tnI, wm, wn, 0, lane, e);
+}
+
+// BF16 products read from the stored operands (bf16x3, bf16x6; no pack launch), for problems
+// whose outputs are too few to amortise a pack (tall-skinny shapes, small m and n). A
+// workgroup of four waves computes a 128 x 128 block of all three real products over one K
+// partition (stages kSteps s / parts .. kSteps (s+1) / parts), each wave a 64 x 64 block: it
+// loads op(A)(i, k) and op(B)(k, j) as stored (buffer loads: uniform base, fixed 32-bit lane
+// offset, small uniform offset), forms the plane values in registers (the pack's values,
+// conjugation as a sign folded into an FMA), splits them into BF16 pieces with bf16_split8
+// (the pack's split), and issues per accumulator the same MFMAs in the same order as the
+// separate-products kernel on packed planes: stage by stage, the terms in TB/TA order. Its
+// output planes (3 s + q, ld = mPad) are therefore the same bits, and merge_kernel combines
+// them the same way. A lane holds k = 8 (lane / 32) + e (e = 0..7) of each 16-deep stage:
+// a k-contiguous operand gives it 64 contiguous bytes (four 16-byte loads), an m- or
+// n-contiguous one eight loads of one k, rows adjacent across lanes. Rows past m (columns past
+// n) read the last one and only fill the planes' padding; the stage reaching past K is
+// peeled after the ring, clamped and zero-masked. The operand layouts are compile-time
+// variants of the k loop (a branch joining loads in flight would make the compiler wait for
+// all of them). The host sends a problem here only if every lane offset fits in 31 bits.
+template <int NP, int ID>
+__global__ __launch_bounds__(256, 1) void gemm3m_bf16_raw_products_kernel(GemmArgs g)
+{
+    constexpr int TM = 2, TN = 2, NSET = 2;
+    constexpr int NTERM = NP == 2 ? 3 : 6;
+    constexpr int TB[6] = {1, NP == 2 ? 0 : 2, 0, 1, 0, 0};
+    constexpr int TA[6] = {NP == 3 ? 1 : 0, NP == 2 ? 1 : 0, NP == 2 ? 0 : 2, 0, 1, 0};
+    typedef float rawf2 __attribute__((ext_vector_type(2)));
+    typedef float rawf4 __attribute__((ext_vector_type(4), aligned(8)));
+    typedef __attribute__((address_space(1))) const rawf2 gf2;
+
+    int sp, tmI, tnI;
+    map_tile(blockIdx.x, gridDim.x, g.tilesM, g.tilesN, sp, tmI, tnI);
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i0 = tmI * 128 + (wave & 1) * 64, j0 = tnI * 128 + (wave >> 1) * 64;
+    // a 64 x 64 block wholly past m or n holds only padding, which nothing reads (and its row
+    // or column base would lie past the operand): the wave has nothing to do
+    if (i0 >= g.m || j0 >= g.n) return;
+    const int ks0 = (int)((long long)g.kSteps * sp / g.parts), ks1 = (int)((long long)g.kSteps * (sp + 1) / g.parts);
+    const int K = g.src.k;
+    const bool aKc = g.src.opA != 111, bKc = g.src.opB == 111;
+    const long long aStepB = aKc ? 8 : g.src.lda * 8, bStepB = bKc ? 8 : g.src.ldb * 8;
+    const char* aTile = reinterpret_cast<const char*>(g.src.A) + (aKc ? (long long)i0 * g.src.lda * 8 : (long long)i0 * 8);
+    const char* bTile = reinterpret_cast<const char*>(g.src.B) + (bKc ? (long long)j0 * g.src.ldb * 8 : (long long)j0 * 8);
+    const float aSign = g.src.opA == 113 ? -1.0f : 1.0f, bSign = g.src.opB == 113 ? -1.0f : 1.0f;
+    unsigned aOff[TM], bOff[TN];
+#pragma unroll
+    for (int b = 0; b < TM; ++b) {
+        const long long d = min(i0 + 32 * b + (lane & 31), g.m - 1) - i0;
+        aOff[b] = (unsigned)((aKc ? d * g.src.lda * 8 : d * 8) + 8 * half * aStepB);
+    }
+#pragma unroll
+    for (int a = 0; a < TN; ++a) {
+        const long long d = min(j0 + 32 * a + (lane & 31), g.n - 1) - j0;
+        bOff[a] = (unsigned)((bKc ? d * g.src.ldb * 8 : d * 8) + 8 * half * bStepB);
+    }
+    auto rsrc = [](const char* base) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)0xffffffff, 0x00020000);
+    };
+    auto bl2 = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(rawf2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    };
+    auto bl4 = [](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+        return __builtin_bit_cast(rawf4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    };
+    struct Set {
+        rawf2 a[TM][8], b[TN][8];  // element e: k = 16 stage + 8 (lane / 32) + e
+    };
+    auto load = [&](Set& S, int ks, auto aTag, auto bTag) {
+        constexpr bool AKC = decltype(aTag)::value, BKC = decltype(bTag)::value;
+        const __amdgpu_buffer_rsrc_t ar = rsrc(aTile + (long long)ks * 16 * aStepB);
+        const __amdgpu_buffer_rsrc_t br = rsrc(bTile + (long long)ks * 16 * bStepB);
+#pragma unroll
+        for (int b = 0; b < TM; ++b)
+#pragma unroll
+            for (int e = 0; e < 8; e += (AKC ? 2 : 1)) {
+                if constexpr (AKC) {
+                    const rawf4 w = bl4(ar, aOff[b], 8 * e);
+                    S.a[b][e] = rawf2{w[0], w[1]};
+                    S.a[b][e + 1] = rawf2{w[2], w[3]};
+                } else {
+                    S.a[b][e] = bl2(ar, aOff[b], (unsigned)(e * aStepB));
+                }
+            }
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int e = 0; e < 8; e += (BKC ? 2 : 1)) {
+                if constexpr (BKC) {
+                    const rawf4 w = bl4(br, bOff[a], 8 * e);
+                    S.b[a][e] = rawf2{w[0], w[1]};
+                    S.b[a][e + 1] = rawf2{w[2], w[3]};
+                } else {
+                    S.b[a][e] = bl2(br, bOff[a], (unsigned)(e * bStepB));
+                }
+            }
+    };
+    // the stage reaching past K: k clamped to K - 1 per element (the lane's k offset 8 (lane /
+    // 32) sits in the lane offsets; a clamped k, per lane, takes it out again)
+    auto loadClamped = [&](Set& S, int ks) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const long long kr = min(ks * 16 + 8 * half + e, K - 1) - 8 * half;
+            const char* ar = aTile + kr * aStepB;
+            const char* br = bTile + kr * bStepB;
+#pragma unroll
+            for (int b = 0; b < TM; ++b) S.a[b][e] = *(gf2*)(ar + aOff[b]);
+#pragma unroll
+            for (int a = 0; a < TN; ++a) S.b[a][e] = *(gf2*)(br + bOff[a]);
+        }
+    };
+
+    f32x16 acc[3][TN][TM];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[q][a][b][r] = 0.0f;
+
+    // plane values with the conjugation sign folded in (s * im is exact, so fma(s, im, x)
+    // rounds as x + s * im does): plane_value(role, ID, q, re, s * im)
+    auto planeA = [&](int q, float re, float im) {
+        if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? aSign * im : fmaf(aSign, im, re));
+        return q == 0 ? fmaf(aSign, im, re) : (q == 1 ? re : aSign * im);  // (Ar+Ai, Ar, Ai)
+    };
+    auto planeB = [&](int q, float re, float im) {
+        if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? bSign * im : fmaf(bSign, im, re));
+        return q == 0 ? re : (q == 1 ? fmaf(bSign, im, -re) : fmaf(bSign, im, re));  // (Br, Bi-Br, Br+Bi)
+    };
+    auto mfma = [&](const Set& S, int ks, auto maskTag) {
+        constexpr bool MASK = decltype(maskTag)::value;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            u32x4 ah[NP][TM], bh[NP][TN];
+#pragma unroll
+            for (int b = 0; b < TM; ++b) {
+                float x[8];
+                u32x4 w[NP];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const bool kin = !MASK || ks * 16 + 8 * half + e < K;
+                    x[e] = kin ? planeA(q, S.a[b][e].x, S.a[b][e].y) : 0.0f;
+                }
+                bf16_split8<NP>(x, w);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) ah[p][b] = w[p];
+            }
+#pragma unroll
+            for (int a = 0; a < TN; ++a) {
+                float x[8];
+                u32x4 w[NP];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const bool kin = !MASK || ks * 16 + 8 * half + e < K;
+                    x[e] = kin ? planeB(q, S.b[a][e].x, S.b[a][e].y) : 0.0f;
+                }
+                bf16_split8<NP>(x, w);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) bh[p][a] = w[p];
+            }
+#pragma unroll
+            for (int t = 0; t < NTERM; ++t)
+#pragma unroll
+                for (int a = 0; a < TN; ++a)
+#pragma unroll
+                    for (int b = 0; b < TM; ++b) acc[q][a][b] = mfma16(bh[TB[t]][a], ah[TA[t]][b], acc[q][a][b], false);
+        }
+    };
+    typedef std::integral_constant<bool, false> Off;
+    typedef std::integral_constant<bool, true> On;
+
+    // full stages through the ring (NSET register sets, unconditional clamped look-ahead
+    // loads, scheduling barriers keeping each stage's loads ahead of its MFMAs), then the
+    // stages reaching past K, clamped and masked
+    const int nst = ks1 - ks0;
+    const int nFull = min(nst, max(0, K / 16 - ks0));
+    auto ring = [&](auto aTag, auto bTag) {
+        Set S[NSET];
+        if (nFull > 0) {
+#pragma unroll
+            for (int u = 0; u + 1 < NSET; ++u) load(S[u], ks0 + min(u, nFull - 1), aTag, bTag);
+        }
+        int st = 0;
+        for (; st + NSET <= nFull; st += NSET) {
+#pragma unroll
+            for (int u = 0; u < NSET; ++u) {
+                load(S[(u + NSET - 1) % NSET], ks0 + min(st + u + NSET - 1, nFull - 1), aTag, bTag);
+                __builtin_amdgcn_sched_barrier(0);
+                mfma(S[u], ks0 + st + u, Off());
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u + 1 < NSET; ++u)
+            if (st + u < nFull) mfma(S[u], ks0 + st + u, Off());
+    };
+    if (aKc) {
+        if (bKc)
+            ring(On(), On());
+        else
+            ring(On(), Off());
+    } else {
+        if (bKc)
+            ring(Off(), On());
+        else
+            ring(Off(), Off());
+    }
+    for (int t = nFull; t < nst; ++t) {
+        Set T;
+        loadClamped(T, ks0 + t);
+        mfma(T, ks0 + t, On());
+    }
+
+    // product q of partition sp into plane 3 sp + q (ld = mPad)
+    const int iBase = i0 + (lane & 31), jBase = j0 + 4 * half;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float* P = g.partial + (size_t)(3 * sp + q) * g.partStride;
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int i = iBase + b * 32;
+                    const int j = jBase + a * 32 + 8 * (r >> 2) + (r & 3);
+                    P[(size_t)j * g.mPad + i] = acc[q][a][b][r];
+                }
+    }
+}
+
+// g.tilesM / g.tilesN count 128 x 128 tiles, g.parts K partitions; g.partial as for the
+// separate-products kernel (3 g.parts planes of mPad x nPad).
+hipError_t launch_gemm_bf16_raw_products(const GemmArgs& g, int pieces, hipStream_t s)
+{
+    const int blocks = g.parts * g.tilesM * g.tilesN;
+    const bool std3m = g.identity == ID_STANDARD;
+    if (pieces == 2) {
+        if (std3m)
+            hipLaunchKernelGGL((gemm3m_bf16_raw_products_kernel<2, ID_STANDARD>), dim3(blocks), dim3(256), 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm3m_bf16_raw_products_kernel<2, ID_FUSED>), dim3(blocks), dim3(256), 0, s, g);
+    } else if (pieces == 3) {
+        if (std3m)
+            hipLaunchKernelGGL((gemm3m_bf16_raw_products_kernel<3, ID_STANDARD>), dim3(blocks), dim3(256), 0, s, g);
+        else
+            hipLaunchKernelGGL((gemm3m_bf16_raw_products_kernel<3, ID_FUSED>), dim3(blocks), dim3(256), 0, s, g);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // g.tilesM / g.tilesN count 256 x 256 tiles (mPad, nPad multiples of 256); g.partial holds

@@ -117,6 +117,7 @@ struct Plan {
     int correction = 0;
     int identity = ID_FUSED;
     int variant = 0;      // product-kernel variant: 0 four waves per tile, 1 eight waves
+    bool raw = false;     // path 4 on BF16: products read from the stored operands (no pack)
     int mPad = 0, nPad = 0, kPad = 0, kSteps = 0;
     size_t aPlaneBytes = 0, bPlaneBytes = 0;
     int planesA = 3, planesB = 3;  // planes * pieces per operand

@@ -26,6 +26,8 @@ hipError_t launch_fill_u64(unsigned long long* p, long long words, hipStream_t s
 hipError_t launch_pack_grouped(const GroupPack* items, int nitems, long long blocks, unsigned int* queue,
                                hipStream_t s);
 hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, hipStream_t s);
+hipError_t launch_energy_partial_pair(const PackArgs& a, const PackArgs& b, hipStream_t s);
+hipError_t launch_gemm_bf16_raw_products(const GemmArgs& g, int pieces, hipStream_t s);
 hipError_t launch_gemm_f32_grouped_quads(const GroupProblem* probs, const GroupTile* tiles, int nquads,
                                          unsigned int* queue, int blocks, bool raw, hipStream_t s);
 hipError_t launch_merge_grouped(const GroupMerge* items, int nitems, int maxOutputs, hipStream_t s);

@@ -111,21 +111,14 @@ __device__ __forceinline__ void pack_tile(const PackArgs& a, int rt, int ct, flo
                     x[s] = plane_value(a.role, a.identity, q, w.x, w.y);
                 }
                 if (a.format == FMT_BF16) {
-                    for (int p = 0; p < a.pieces; ++p) {
-                        uint4 w;
-                        uint16_t h[8];
-#pragma unroll
-                        for (int s = 0; s < 8; ++s) {
-                            h[s] = f32_to_bf16_rne(x[s]);
-                            x[s] = x[s] - bf16_to_f32(h[s]);  // exact: remainder of the rounding
-                        }
-                        w.x = h[0] | ((uint32_t)h[1] << 16);
-                        w.y = h[2] | ((uint32_t)h[3] << 16);
-                        w.z = h[4] | ((uint32_t)h[5] << 16);
-                        w.w = h[6] | ((uint32_t)h[7] << 16);
-                        reinterpret_cast<uint4*>(static_cast<char*>(a.planes) +
-                                                 (size_t)(q * a.pieces + p) * a.planeBytes)[idx] = w;
-                    }
+                    u32x4 w[3];
+                    if (a.pieces == 3)
+                        bf16_split8<3>(x, w);
+                    else
+                        bf16_split8<2>(x, *reinterpret_cast<u32x4(*)[2]>(w));
+                    for (int p = 0; p < a.pieces; ++p)
+                        reinterpret_cast<u32x4*>(static_cast<char*>(a.planes) +
+                                                 (size_t)(q * a.pieces + p) * a.planeBytes)[idx] = w[p];
                 } else {
                     const int kb = (c0 + cl) >> 7;
                     const int e = a.scaleExp[((size_t)q * (a.mnPad >> 7) + (r0 >> 7)) * ((a.kPad + 127) >> 7) + kb];
@@ -188,29 +181,16 @@ __global__ __launch_bounds__(256) void pack_grouped_kernel(const GroupPack* item
     pack_tile(p, local % rowTiles, local / rowTiles, tile);
 }
 
-// Energy partials of grouped members whose energies come before the product kernel (K-split
-// members, whose screen runs in the merge): the pack kernel's partials (thread (row, h): 32
-// elements of the 64 x 64 tile in the half-sum order of energy_acc), computed without the LDS tile or
-// any plane writes. A thread issues its 32 loads at once: rows adjacent in memory make each
-// load a 512-byte run across the 64 threads of a half; k-contiguous rows give each thread
-// four runs of 32 contiguous bytes per 16 k. Out-of-range rows and k count as zero (clamped
-// addresses, no branch around a load). 128 threads per tile.
-__global__ __launch_bounds__(128) void energy_partial_grouped_kernel(const GroupPack* items, int nitems)
+// Energy partials straight from the stored operands, for plans that read them as stored (the
+// K-split members of a grouped batch, whose screen runs in the merge; the BF16 products read
+// as stored): the pack kernel's partials (thread (row, h): 32 elements of the 64 x 64 tile in
+// the half-sum order of energy_acc), computed without the LDS tile or any plane writes. A
+// thread issues its 32 loads at once: rows adjacent in memory make each load a 512-byte run
+// across the 64 threads of a half; k-contiguous rows give each thread four runs of 32
+// contiguous bytes per 16 k. Out-of-range rows and k count as zero (clamped addresses, no
+// branch around a load). 128 threads per tile.
+__device__ __forceinline__ void energy_partial_tile(const PackArgs& a, int rt, int ct)
 {
-    const long long b = blockIdx.x;
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].firstBlock <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const int idx = __builtin_amdgcn_readfirstlane(lo);
-    const PackArgs a = items[idx].a;
-    const int local = (int)(b - items[idx].firstBlock);
-    const int rowTiles = a.mnPad / 64;
-    const int rt = local % rowTiles, ct = local / rowTiles;
     const int tid = threadIdx.x, row = tid & 63, h = tid >> 6;
     const int r = rt * 64 + row, c0 = ct * 64;
     const int rr = min(r, a.mn - 1);
@@ -253,6 +233,43 @@ __global__ __launch_bounds__(128) void energy_partial_grouped_kernel(const Group
     }
     const int kTiles = (a.kPad + 63) / 64;
     a.energyPartial[((size_t)h * kTiles + ct) * a.mnPad + r] = t;
+}
+
+// Grouped: workgroup b finds its (problem, operand) item by binary search over the items'
+// first-workgroup prefix.
+__global__ __launch_bounds__(128) void energy_partial_grouped_kernel(const GroupPack* items, int nitems)
+{
+    const long long b = blockIdx.x;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].firstBlock <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int idx = __builtin_amdgcn_readfirstlane(lo);
+    const PackArgs a = items[idx].a;
+    const int local = (int)(b - items[idx].firstBlock);
+    const int rowTiles = a.mnPad / 64;
+    energy_partial_tile(a, local % rowTiles, local / rowTiles);
+}
+
+// One problem: operands a (blocks 0 .. blocksA - 1) and b.
+__global__ __launch_bounds__(128) void energy_partial_kernel(PackArgs a, PackArgs b, int blocksA)
+{
+    const bool isA = (int)blockIdx.x < blocksA;
+    const PackArgs& p = isA ? a : b;
+    const int local = isA ? blockIdx.x : blockIdx.x - blocksA;
+    const int rowTiles = p.mnPad / 64;
+    energy_partial_tile(p, local % rowTiles, local / rowTiles);
+}
+
+hipError_t launch_energy_partial_pair(const PackArgs& a, const PackArgs& b, hipStream_t s)
+{
+    const int na = pack_blocks(a), nb = pack_blocks(b);
+    hipLaunchKernelGGL(energy_partial_kernel, dim3(na + nb), dim3(128), 0, s, a, b, na);
+    return hipGetLastError();
 }
 
 hipError_t launch_energy_partial_grouped(const GroupPack* items, int nitems, long long blocks, hipStream_t s)

@@ -140,7 +140,8 @@ typedef enum {
 typedef struct {
     int precision;  /* aclblasCgemm3mPrecision_t */
     int tile;       /* 0 auto, 64 or 128: output tile edge */
-    int kSplit;     /* 0 auto, else 1..16 K partitions reduced in fixed order */
+    int kSplit;     /* 0 auto, else 1..16 K partitions reduced in fixed order (1..128 for the
+                       BF16 separate products, variants 4 and 7) */
     int correction; /* 0 auto (on when k >= 1024), 1 on: flagged outputs recomputed in FP64,
                        2 on: flagged outputs recomputed in the reference BLAS's FP32 order
                        (Netlib cgemm, bit-identical to it for those outputs), -1 off */
@@ -153,7 +154,11 @@ typedef struct {
                        5 one-launch small-problem kernel reading A and B directly, no
                        workspace (FP32 3M, no K split); 6 separate products with the
                        fragments loaded once per workgroup into an LDS ring by direct
-                       global-to-LDS loads (16-bit precisions; measured slower than 4) */
+                       global-to-LDS loads (16-bit precisions; measured slower than 4);
+                       7 separate products read from the stored A and B, no pack launch
+                       (BF16 precisions, kSplit K partitions; chosen by default for
+                       tall-skinny and small outputs; the same bytes as 4 with the same
+                       partitions). Variants 4 and 7 take kSplit on BF16 precisions only. */
 } aclblasCgemm3mConfig;
 
 /* For callers that choose the arithmetic or pin the plan (bench, tests, autotuner, the

@@ -219,6 +219,59 @@ def test_kernel_variants_identical(gpu, prec, variants, parts):
             v, variants[0], int(np.count_nonzero(o.view(np.uint64) != outs[0].view(np.uint64))))
 
 
+RAW_BF16_CASES = [
+    # (m, n, k, ops, parts): ragged edges, all op layouts, K partitions, a k tail
+    (200, 250, 3000, "NN", 4), (256, 256, 4096, "TC", 0), (130, 70, 1000, "CN", 1), (64, 300, 2048, "NT", 3),
+    (300, 200, 777, "TT", 2), (128, 128, 8192, "CC", 0),
+]
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16x6"])
+@pytest.mark.parametrize("case", RAW_BF16_CASES, ids=["%dx%dx%d-%s-p%d" % c for c in RAW_BF16_CASES])
+def test_bf16_raw_products_identical(gpu, prec, case):
+    """BF16 products read from the stored operands (variant 7) against the same products on
+    planes packed first (variant 4), with the same K partitions: the plane values and BF16
+    pieces are formed with the pack's arithmetic and every accumulator sees the same MFMAs in
+    the same order, so the outputs are byte-identical; bf16x6's also passes the oracle check
+    (bf16x3 drops the lo*lo terms and can miss the reference verifier's FP32 match ratio).
+    One case scales a row block of A down to FP32 denormals (kept by both splits)."""
+    import cgemm3m
+
+    m, n, k, ops, parts = case
+    ta, tb = ops[0], ops[1]
+    lda = (m if ta == "N" else k) + 3
+    ldb = (k if tb == "N" else n) + 1
+    ldc = m + 2
+    seed = 20261204 + m + k
+    A, B, C = inputs(ta, tb, m, n, k, lda, ldb, ldc, seed)
+    denormal = ops == "TT"
+    if denormal:
+        # op(A) rows 0..39 (A stored k x m, so its first 40 columns) scaled to FP32 denormals:
+        # both splits keep them; BF16 pieces of denormals carry few bits, so no oracle check
+        A = A.copy()
+        A[: 40 * lda] *= 1e-39
+    outs = []
+    for v in (7, 4):  # the raw form first: its plan's partitions (parts 0: its own choice) pin 4's
+        cfg = cgemm3m.config(prec, k_split=parts, variant=v)
+        print("case raw-bf16", prec, case, "variant", v, "parts", parts)
+        sys.stdout.flush()
+        if v == 4 and not denormal and prec == "bf16x6":  # bf16x3 misses the verifier at times
+            out, _ = _case(gpu, ta, tb, m, n, k, ALPHA2, BETA2, lda, ldb, ldc, seed, cfg=cfg,
+                           label="raw-%s-v%d" % (prec, v))
+        elif v == 4:
+            st, out = run_gpu(gpu, ta, tb, m, n, k, ALPHA2, A, lda, B, ldb, BETA2, C, ldc, cfg)
+            assert st == 0
+        else:
+            st, out = run_gpu(gpu, ta, tb, m, n, k, ALPHA2, A, lda, B, ldb, BETA2, C, ldc, cfg)
+            assert st == 0
+            plan = gpu.last_plan(sync=True)
+            assert plan["path"] == 4 and plan["tile"] == 128, plan
+            parts = plan["parts"]
+        outs.append(out)
+    diff = int(np.count_nonzero(outs[1].view(np.uint64) != outs[0].view(np.uint64)))
+    assert diff == 0, "%d outputs differ between the raw and packed products" % diff
+
+
 @pytest.mark.parametrize("parts", [1, 3])
 def test_tile_sizes_identical(gpu, parts):
     """64 x 64 tiles through LDS (four waves of 32 x 32) and 128 x 128 tiles loading
@@ -272,6 +325,8 @@ INT_CASES = [
     ("bf16x3-products-ksplit5", 300, 260, 2048, "CN", dict(precision="bf16x3", variant=4, k_split=5)),
     ("bf16x6-products-ksplit3", 260, 300, 1024, "TC", dict(precision="bf16x6", variant=4, k_split=3)),
     ("bf16x3-tall-skinny-auto", 200, 256, 8192, "NT", dict(precision="bf16x3")),
+    ("bf16x3-raw-ksplit4", 200, 250, 3000, "CT", dict(precision="bf16x3", variant=7, k_split=4)),
+    ("bf16x6-raw", 130, 70, 1000, "TN", dict(precision="bf16x6", variant=7)),
 ]
 
 
