@@ -223,20 +223,12 @@ __device__ __forceinline__ double2 load_op(const float2* X, long long ld, int op
     return make_double2((double)v.x, op == 113 ? -(double)v.y : (double)v.y);
 }
 
-__device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
+// The flagged outputs of bitmap word w (bits = its value), recomputed by the whole workgroup.
+__device__ __forceinline__ void recompute_word(const RecomputeArgs& a, long long w, unsigned long long bits,
+                                               const float2* A, const float2* B, float2* C, double (*part)[2],
+                                               int tid, int lane, int wave)
 {
-    __shared__ double part[4][2];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const float2* A = reinterpret_cast<const float2*>(a.A);
-    const float2* B = reinterpret_cast<const float2*>(a.B);
-    float2* C = reinterpret_cast<float2*>(a.C);
-    // Each workgroup takes one bitmap word at a time (64 outputs; flagged outputs are rare, so
-    // spreading the words over many workgroups spreads them too) and handles its set bits in
-    // increasing order, all 256 threads on one output.
-    for (long long w = blockIdx.x; w < a.words; w += gridDim.x) {
-        const unsigned long long word = a.bitmap[w];
-        unsigned long long bits = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(word >> 32)) << 32) |
-                                  (unsigned)__builtin_amdgcn_readfirstlane((int)word);
+    {
         if (a.netlib) {
             // one flagged output per thread, each in the reference BLAS's sequential FP32 order
             if (tid < 64 && ((bits >> tid) & 1ull)) {
@@ -248,7 +240,7 @@ __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
                                                          make_float2(a.betaRf, a.betaIf), a.betaZero != 0, c0);
                 if (a.counter != nullptr) atomicAdd(a.counter, 1u);
             }
-            continue;
+            return;
         }
         while (bits != 0ull) {
             const int b = __ffsll((long long)bits) - 1;
@@ -311,6 +303,38 @@ __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
     }
 }
 
+__device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
+{
+    __shared__ double part[4][2];
+    __shared__ unsigned long long waveMask[4], wordVal[256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float2* A = reinterpret_cast<const float2*>(a.A);
+    const float2* B = reinterpret_cast<const float2*>(a.B);
+    float2* C = reinterpret_cast<float2*>(a.C);
+    // Each workgroup scans 256 consecutive bitmap words at a time (one coalesced load, a ballot
+    // per wave) and handles the set bits of the nonzero ones in increasing order, all 256
+    // threads on one output (flagged outputs are rare, so they spread over the workgroups).
+    for (long long base = (long long)blockIdx.x * 256; base < a.words; base += (long long)gridDim.x * 256) {
+        const long long wi = base + tid;
+        const unsigned long long mine = wi < a.words ? a.bitmap[wi] : 0ull;
+        const unsigned long long live = __ballot(mine != 0ull);
+        wordVal[tid] = mine;
+        if (lane == 0) waveMask[wave] = live;
+        __syncthreads();
+        for (int wv = 0; wv < 4; ++wv) {
+            unsigned long long mk = waveMask[wv];
+            while (mk != 0ull) {
+                const int src = __ffsll((long long)mk) - 1;
+                mk &= mk - 1ull;
+                const long long w = base + wv * 64 + src;
+                recompute_word(a, w, wordVal[wv * 64 + src], A, B, C, part, tid, lane, wave);
+            }
+        }
+        __syncthreads();  // waveMask and wordVal are rewritten by the next group of words
+    }
+}
+
+
 __global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a) { recompute_body(a); }
 
 // Grouped batch: blockIdx.y picks the member (one with the rounding correction on).
@@ -322,7 +346,7 @@ __global__ __launch_bounds__(256) void recompute_grouped_kernel(const RecomputeA
 
 hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s)
 {
-    long long bx = maxWords;  // a bitmap word per workgroup
+    long long bx = (maxWords + 255) / 256;  // 256 bitmap words per workgroup and step
     if (bx > 4096) bx = 4096;
     if (bx < 1) bx = 1;
     if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
@@ -332,8 +356,8 @@ hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)
 {
-    long long blocks = a.words;  // a bitmap word per workgroup
-    if (blocks > 16384) blocks = 16384;
+    long long blocks = (a.words + 255) / 256;  // 256 bitmap words per workgroup and step
+    if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
     return hipGetLastError();

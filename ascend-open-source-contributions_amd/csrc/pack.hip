@@ -476,10 +476,13 @@ __device__ __forceinline__ void energy_body(const EnergyArgs& a)
     // the two half partials' tile sums, each in tile order, then half 0 + half 1
     const size_t hA = (size_t)a.kTiles * a.mnPadA, hB = (size_t)a.kTiles * a.mnPadB;
     if (a.kTiles <= kLongChain) {
-        for (long long r = t; r < a.mnPadA; r += stride)
-            a.eA[r] = ordered_sum2(a.partialA + r, a.partialA + hA + r, a.mnPadA, a.kTiles);
-        for (long long r = t; r < a.mnPadB; r += stride)
-            a.eB[r] = ordered_sum2(a.partialB + r, a.partialB + hB + r, a.mnPadB, a.kTiles);
+        for (long long r = t; r < (long long)a.mnPadA + a.mnPadB; r += stride) {
+            if (r < a.mnPadA)
+                a.eA[r] = ordered_sum2(a.partialA + r, a.partialA + hA + r, a.mnPadA, a.kTiles);
+            else
+                a.eB[r - a.mnPadA] = ordered_sum2(a.partialB + (r - a.mnPadA), a.partialB + hB + (r - a.mnPadA),
+                                                  a.mnPadB, a.kTiles);
+        }
     } else {
         const int lane = threadIdx.x & 63;
         const long long rows = (long long)a.mnPadA + a.mnPadB;
@@ -518,7 +521,7 @@ hipError_t launch_energy_grouped(const EnergyArgs* items, int nitems, long long 
 
 long long energy_work(int mnPadA, int mnPadB, int kTiles, long long bitmapWords)
 {
-    long long work = mnPadA > mnPadB ? mnPadA : mnPadB;  // a thread per row
+    long long work = (long long)mnPadA + mnPadB;  // a thread per row
     if (kTiles > kLongChain) work = 64LL * ((long long)mnPadA + mnPadB);  // a wave per row
     return bitmapWords > work ? bitmapWords : work;
 }
