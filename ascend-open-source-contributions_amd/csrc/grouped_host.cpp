@@ -269,6 +269,22 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
     // dimensions of 8 bytes from the tile's base); wider leading dimensions take the packed form.
     for (int i = 0; i < count && withPointers; ++i)
         if ((long long)std::max(lda[i], ldb[i]) * 8 * 68 >= (1LL << 31)) L.raw = false;
+    // The tuned table gives a class to the one-launch kernel from batches of that class alone.
+    // In a batch whose quad-kernel members carry at least as much work, its persistent
+    // workgroups hold every CU until their queue empties, so the one-launch launches would in
+    // effect run after it: such members join the quad kernel instead (64 x 64 tiles, no K
+    // split), where they fill its queue. Unless tiles, K splits or the one-launch kernel are
+    // pinned; the same in both forms (variant 7 included), so both run the same plans.
+    bool absorb = false;
+    if (over == nullptr || ((over->variant == 0 || over->variant == 7) && over->tile == 0 && over->kSplit == 0)) {
+        double quadWork = 0.0, smallWork = 0.0;
+        for (int i = 0; i < count; ++i) {
+            if (m[i] <= 0 || n[i] <= 0 || k[i] <= 0) continue;
+            const double w = (double)m[i] * n[i] * k[i];
+            (grouped_member_config(m[i], n[i], k[i], over).variant == 5 ? smallWork : quadWork) += w;
+        }
+        absorb = smallWork > 0.0 && quadWork >= smallWork;
+    }
     for (int i = 0; i < count; ++i) {
         const aclblasStatus_t st =
             validate(h, ta[i], tb[i], m[i], n[i], k[i], &alpha[i], withPointers ? lda[i] : std::max(1, ta[i] != ACLBLAS_OP_N ? k[i] : m[i]),
@@ -288,7 +304,12 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             if (!(p.beta.real == 1.0f && p.beta.imag == 0.0f)) L.scaleOnly.push_back(p);
             continue;
         }
-        const aclblasCgemm3mConfig cfg = grouped_member_config(p.m, p.n, p.k, over);
+        aclblasCgemm3mConfig cfg = grouped_member_config(p.m, p.n, p.k, over);
+        if (absorb && cfg.variant == 5) {
+            cfg.tile = 64;
+            cfg.kSplit = 1;
+            cfg.variant = 1;
+        }
         Plan pl;
         if (!make_plan(p, &cfg, h->cus, pl, op)) return ACLBLAS_STATUS_INVALID_VALUE;
         if (pl.path == 5) {  // the one-launch kernel: no pack, no workspace

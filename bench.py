@@ -51,7 +51,8 @@ def main():
                          "WRITE_SIZE), nothing timed")
     ap.add_argument("--grouped-only", action="store_true",
                     help="for kernel traces: only the 512-problem grouped batch (one call, its hipGraph "
-                         "replay, the per-problem loop); prints its figures as one JSON line")
+                         "replay, the per-problem loop); prints its figures as one JSON line. With "
+                         "--profile-run: three grouped calls, then the 1 GiB copy, nothing timed")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path here (across GPUs the "
@@ -67,7 +68,17 @@ def main():
     dev = torch.device("cuda:0")
     if args.grouped_only:
         hg = cgemm3m.Handle(stream=torch.cuda.current_stream())
-        print(json.dumps({"grouped_512": grouped_bench(hg, torch, cgemm3m, dev)}), flush=True)
+        if args.profile_run:
+            probs, keep, _ = grouped_problems(torch, cgemm3m, dev)
+            for _ in range(3):
+                cgemm3m.grouped(hg, probs)
+            src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            log("profile run done: 3 grouped calls of 512 problems, then a 1 GiB copy")
+        else:
+            print(json.dumps({"grouped_512": grouped_bench(hg, torch, cgemm3m, dev)}), flush=True)
         hg.close()
         return
     gen = torch.Generator(device=dev)
@@ -297,12 +308,9 @@ def autograd_bench(torch, cgemm3m, dev, args, m=2048, k=1024, n=4096, seed=777):
     return out
 
 
-def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
+def grouped_problems(torch, cgemm3m, dev, count=512, seed=4242):
     """512 uneven problems (m, n in 16*{2..32}, k in 64*{1..32}, ops uniform over {N,T,C}^2,
-    alpha = 1, beta = 0): one grouped call, the same call replayed from a hipGraph, and a
-    loop of one aclblasCgemm3m call per problem, both as the plain call (which turns the
-    rounding-correction screen on for k >= 1024) and with the correction off, as grouped
-    members run. Complex TFLOP/s counts sum(8 m n k)."""
+    alpha = 1, beta = 0), seeded; returns (problems, buffers to keep alive, complex flops)."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -323,6 +331,15 @@ def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
         probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=1.0, A=cgemm3m.torch_ptr(A), lda=lda,
                           B=cgemm3m.torch_ptr(B), ldb=ldb, beta=0.0, C=cgemm3m.torch_ptr(C), ldc=m))
         flops += 8.0 * m * n * k
+    return probs, keep, flops
+
+
+def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
+    """The grouped_problems batch: one grouped call, the same call replayed from a hipGraph,
+    and a loop of one aclblasCgemm3m call per problem, both as the plain call (which turns the
+    rounding-correction screen on for k >= 1024) and with the correction off, as grouped
+    members run. Complex TFLOP/s counts sum(8 m n k)."""
+    probs, keep, flops = grouped_problems(torch, cgemm3m, dev, count, seed)
     stream = torch.cuda.current_stream()
 
     def timed(fn):
