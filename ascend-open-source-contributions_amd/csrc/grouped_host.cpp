@@ -710,6 +710,17 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         const EnergyArgs* dEnergy = reinterpret_cast<const EnergyArgs*>(dDesc + offEnergy);
         const bool fork = L.nProb > 0 && (!L.small.empty() || !sums.empty());
         hipStream_t ss = s;
+        // Raw form: the tile-queue reset and bitmap clears go first, before the fork, so that
+        // the quad kernel right behind them does not wait for CUs the second stream's kernels
+        // took first.
+        if (e == hipSuccess && L.raw && L.nProb > 0) {
+            if (L.nCorr > 0) {
+                StageTimer t(handle, ST_ENERGY);
+                e = launch_energy_grouped(dEnergy, (int)L.nCorr, maxEnergyWork, s);  // also the queue
+            } else {
+                e = launch_fill_u64(reinterpret_cast<unsigned long long*>(queue), 1, s);  // the tile queue
+            }
+        }
         if (fork) {
             RelaxedCapture relaxed;
             if (handle->side == nullptr && hipStreamCreateWithFlags(&handle->side, hipStreamNonBlocking) != hipSuccess)
@@ -750,11 +761,8 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         if (e == hipSuccess && !L.raw && L.nPack > 0) {
             StageTimer t(handle, ST_PACK);
             e = launch_pack_grouped(dPack, (int)L.nPack, packBlocks, queue, s);  // also clears the queue
-        } else if (e == hipSuccess && L.nProb > 0 && !(L.raw && L.nCorr > 0)) {
-            e = launch_fill_u64(reinterpret_cast<unsigned long long*>(queue), 1, s);  // the tile queue
         }
-        if (e == hipSuccess && L.nCorr > 0) {
-            // packed form: energy sums and bitmap clears; raw form: bitmap clears and the queue
+        if (e == hipSuccess && !L.raw && L.nCorr > 0) {  // packed form: energy sums and bitmap clears
             StageTimer t(handle, ST_ENERGY);
             e = launch_energy_grouped(dEnergy, (int)L.nCorr, maxEnergyWork, s);
         }
