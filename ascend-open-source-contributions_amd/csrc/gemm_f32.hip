@@ -330,15 +330,19 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm3m_f32_direct_kernel(Ge
 // accumulator equal the packed 64 x 64 tile's: the results are byte-identical to it.
 typedef float rawf2 __attribute__((ext_vector_type(2)));  // one complex64 element (re, im)
 
-template <int ID>
+template <int ID, int TM, int TN>
 struct RawSet {
-    rawf2 a[2][2][4], b[2][2][4];  // [k fragment][32-row / 32-column fragment][k slot]
+    rawf2 a[2][TM][4], b[2][TN][4];  // [k fragment][32-row / 32-column fragment][k slot]
 };
 
-template <int ID>
+// TM (TN): 32-row (32-column) fragments of the block that hold outputs. A block at a member's
+// last row (column) with at most 32 outputs left takes 1 and skips the MFMAs of the fragment
+// that would be padding; the other fragment's accumulators get the same MFMAs in the same
+// order, so its outputs are unchanged.
+template <int ID, int TM, int TN>
 __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int tmI, int tnI)
 {
-    constexpr int TM = 2, TN = 2, KFD = 2, NSET = 2;
+    constexpr int KFD = 2, NSET = 2;
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;
     const int kf0 = 2 * (int)((long long)g.kSteps * part / g.parts);
@@ -391,7 +395,7 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
     // loads, would make the wait for it come at once, so the operand layouts are compile-time
     // (AKC, BKC: op(A), op(B) k-contiguous, whose lane's four k are 32 contiguous bytes, read
     // as two 16-byte loads; the other layout reads one k per load, rows adjacent across lanes).
-    auto load = [&](RawSet<ID>& S, int st, auto aTag, auto bTag) {
+    auto load = [&](RawSet<ID, TM, TN>& S, int st, auto aTag, auto bTag) {
         constexpr bool AKC = decltype(aTag)::value, BKC = decltype(bTag)::value;
         const int sk = (kf0 + st * KFD) * 8;
 #pragma unroll
@@ -426,7 +430,7 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
     };
     // The stage reaching past K: k clamped to K - 1 per element (the lane's k offset sits in the
     // lane offsets; a clamped k, per lane, takes it out of them again), one k per load.
-    auto loadClamped = [&](RawSet<ID>& S, int st) {
+    auto loadClamped = [&](RawSet<ID, TM, TN>& S, int st) {
         const int sk = (kf0 + st * KFD) * 8;
 #pragma unroll
         for (int kfl = 0; kfl < KFD; ++kfl)
@@ -477,7 +481,7 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
 
     // MASK: k past K contributes zeros (both operands, so an Inf or NaN read from the clamped
     // address adds nothing), to the products and to the energies.
-    auto mfma = [&](const RawSet<ID>& S, int st, auto maskTag) {
+    auto mfma = [&](const RawSet<ID, TM, TN>& S, int st, auto maskTag) {
         constexpr bool MASK = decltype(maskTag)::value;
         const int sk = (kf0 + st * KFD) * 8;
         const int pos = (kf0 / KFD + st) & 3;  // stage within its 64-deep k tile
@@ -535,7 +539,7 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
     const int nst = (kf1 - kf0) / KFD;
     const int nFull = min(nst, max(0, (K - kf0 * 8) / (8 * KFD)));
     auto ring = [&](auto aTag, auto bTag) {
-        RawSet<ID> S[NSET];
+        RawSet<ID, TM, TN> S[NSET];
         if (nFull > 0) {
 #pragma unroll
             for (int u = 0; u + 1 < NSET; ++u) load(S[u], min(u, nFull - 1), aTag, bTag);
@@ -566,7 +570,7 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
             ring(Off(), Off());
     }
     for (int t = nFull; t < nst; ++t) {
-        RawSet<ID> Tl;
+        RawSet<ID, TM, TN> Tl;
         loadClamped(Tl, t);
         mfma(Tl, t, On());
     }
@@ -629,8 +633,16 @@ __global__ __launch_bounds__(256, 1) void gemm3m_f32_grouped_quad_kernel(const G
         const int part = __builtin_amdgcn_readfirstlane(gt.part);
         const int tm = __builtin_amdgcn_readfirstlane(gt.tm);
         const int tn = __builtin_amdgcn_readfirstlane(gt.tn);
-        if constexpr (RAW) {
-            f32_tile_raw64<ID_FUSED>(g, part, tm, tn);  // the host sends fused-identity batches only
+        if constexpr (RAW) {  // the host sends fused-identity batches only
+            const bool m2 = g.m - 64 * tm > 32, n2 = g.n - 64 * tn > 32;
+            if (m2 && n2)
+                f32_tile_raw64<ID_FUSED, 2, 2>(g, part, tm, tn);
+            else if (m2)
+                f32_tile_raw64<ID_FUSED, 2, 1>(g, part, tm, tn);
+            else if (n2)
+                f32_tile_raw64<ID_FUSED, 1, 2>(g, part, tm, tn);
+            else
+                f32_tile_raw64<ID_FUSED, 1, 1>(g, part, tm, tn);
         } else if (g.parts > 1) {
             f32_tile_direct<64, 64, true, 1, 1, false, C3M_DIRECT_KFD, C3M_DIRECT_NSET, true>(g, part, tm, tn);
         } else {

@@ -190,6 +190,25 @@ struct GroupedLayout {
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+// How the quad kernel covers one member's output with 64 x 64 blocks: bm x bn blocks; fm (fn)
+// 32-row (32-column) fragments in the last block row (column), 1 when at most 32 outputs are
+// left there; whole 128 x 128 regions of full blocks (the first bmQ x bnQ blocks) go to quads
+// of one region each, the other blocks to quads of four blocks of like cost.
+struct QuadCover {
+    int bm, bn, fm, fn, bmQ, bnQ;
+};
+static QuadCover quad_cover(int m, int n)
+{
+    QuadCover q;
+    q.bm = (m + 63) / 64;
+    q.bn = (n + 63) / 64;
+    q.fm = m - 64 * (q.bm - 1) > 32 ? 2 : 1;
+    q.fn = n - 64 * (q.bn - 1) > 32 ? 2 : 1;
+    q.bmQ = (q.fm == 2 ? q.bm : q.bm - 1) & ~1;
+    q.bnQ = (q.fn == 2 ? q.bn : q.bn - 1) & ~1;
+    return q;
+}
+
 // Workspace of a member the quad kernel reads raw: no planes, only the K-split partial planes
 // and, with the correction on, the bitmap and counter; a K-split member also keeps its energy
 // partials (from the energy-only pass, two halves per k tile) and energies, which the merge
@@ -327,8 +346,9 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
         L.base.push_back(total);
         total = align256(total + L.plans[i].total);
         const Plan& pl = L.plans[i];
-        const size_t bm = (size_t)(L.probs[i].m + 63) / 64, bn = (size_t)(L.probs[i].n + 63) / 64;
-        const size_t whole = (bm / 2) * (bn / 2);
+        const QuadCover qc = quad_cover(L.probs[i].m, L.probs[i].n);
+        const size_t bm = (size_t)qc.bm, bn = (size_t)qc.bn;
+        const size_t whole = (size_t)(qc.bmQ / 2) * (size_t)(qc.bnQ / 2);
         L.nQuad += whole * (size_t)pl.parts;
         loose += (bm * bn - 4 * whole) * (size_t)pl.parts;
         if (pl.parts > 1) {
@@ -579,23 +599,28 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                 recomputes.push_back(r);
                 maxWords = std::max<long long>(maxWords, pl.bitmapWords);
             }
-            // 64 x 64 blocks: whole 128 x 128 regions as quads in the 128 x 128 kernel's wave
-            // order (wave w: block row 2r + w % 2, block column 2c + w / 2); the odd last block
-            // row and column go to the loose list
+            // 64 x 64 blocks: whole 128 x 128 regions of full blocks as quads in the 128 x 128
+            // kernel's wave order (wave w: block row 2r + w % 2, block column 2c + w / 2); the
+            // other blocks (odd last block row or column, and a last block row or column with
+            // at most 32 outputs, which the kernel runs on one 32-wide fragment) go to the loose
+            // list, costed by their MFMA work: k steps x 32 x 32 fragments
             const long long stepsPerPart = (pl.kSteps + pl.parts - 1) / pl.parts;
-            const int bm = (p.m + 63) / 64, bn = (p.n + 63) / 64;
+            const QuadCover qc = quad_cover(p.m, p.n);
+            const int bm = qc.bm, bn = qc.bn;
             for (int part = 0; part < pl.parts; ++part) {
-                for (int tn = 0; tn + 1 < bn; tn += 2)
-                    for (int tm = 0; tm + 1 < bm; tm += 2) {
+                for (int tn = 0; tn < qc.bnQ; tn += 2)
+                    for (int tm = 0; tm < qc.bmQ; tm += 2) {
                         Quad qd;
-                        qd.cost = stepsPerPart;
+                        qd.cost = 4 * stepsPerPart;
                         for (int w = 0; w < 4; ++w) qd.t[w] = GroupTile{(int)i, part, tm + w % 2, tn + w / 2};
                         quads.push_back(qd);
                     }
                 for (int tn = 0; tn < bn; ++tn)
                     for (int tm = 0; tm < bm; ++tm)
-                        if ((bm % 2 == 1 && tm == bm - 1) || (bn % 2 == 1 && tn == bn - 1))
-                            loose.push_back(Cost{stepsPerPart, GroupTile{(int)i, part, tm, tn}});
+                        if (tm >= qc.bmQ || tn >= qc.bnQ) {
+                            const int frags = (tm == bm - 1 ? qc.fm : 2) * (tn == bn - 1 ? qc.fn : 2);
+                            loose.push_back(Cost{frags * stepsPerPart, GroupTile{(int)i, part, tm, tn}});
+                        }
             }
             if (pl.parts > 1) {
                 MergeArgs& a = mg[nm++].a;
