@@ -1,13 +1,15 @@
 // grouped_host.cpp -- aclblasCgemm3mGroupedBatched: many independent complex GEMMs in one
-// packing launch, one persistent product launch per tile size, and (only if some problem
-// is K-split) one merge launch. No reference counterpart (the reference's batched entry is
-// the uniform strided one, ops-blas/blas/gemm/arch35/cgemm_host.cpp:740-782).
+// persistent product launch (operands read as stored, 64 x 64 wave blocks pulled from one
+// queue), with the one-launch small-problem kernel beside it for members the table gives to
+// it, and a merge and a recompute launch when some member needs them. No reference
+// counterpart (the reference's batched entry is the uniform strided one,
+// ops-blas/blas/gemm/arch35/cgemm_host.cpp:740-782).
 //
 // Plans per problem come from the autotuned table (tuned_table.h) with grouped = 1, else a
-// fixed rule: 128 tiles for problems at least 256 x 256, 64 tiles otherwise, no K split
-// (the batch itself fills the CUs). The descriptor block (problem arguments, pack items,
-// tile list, merge items) is built on the host and kept on the device per content, so a
-// repeated call (or a hipGraph replay of one) launches kernels only.
+// fixed rule: 128 tiles for problems at least 256 x 256, 64 tiles otherwise, no K split (the
+// batch itself fills the CUs). The descriptor block (problem arguments, pack items, tile
+// list, merge items) is built on the host and kept on the device per content, so a repeated
+// call (or a hipGraph replay of one) launches kernels only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -270,6 +272,15 @@ aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCge
         cfg.kSplit = 1;
     } else {
         cfg.variant = cfg.tile == 128 ? 3 : 1;
+        // A member with the rounding correction on takes no K split unless the call pins one:
+        // the persistent kernel then forms its energies from its own loads, where a split
+        // member needs an energy pass over its operands beside it (whose workgroups hold CUs
+        // the persistent kernel waits for), a partial-plane merge and a longer recompute
+        // list. The persistent queue balances a batch without the split. (One rule for every
+        // form, so a member's bytes do not depend on the form.)
+        const bool corrected = cfg.correction != 0 ? cfg.correction > 0 : k >= 1024;
+        const bool pinned = over != nullptr && (over->tile != 0 || over->kSplit != 0);
+        if (corrected && !pinned) cfg.kSplit = 1;
     }
     return cfg;
 }
