@@ -334,16 +334,19 @@ def grouped_problems(torch, cgemm3m, dev, count=512, seed=4242):
     return probs, keep, flops
 
 
-def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
+def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242):
     """The grouped_problems batch: one grouped call, the same call replayed from a hipGraph,
-    and a loop of one aclblasCgemm3m call per problem, both as the plain call (which turns the
-    rounding-correction screen on for k >= 1024) and with the correction off, as grouped
-    members run. Complex TFLOP/s counts sum(8 m n k)."""
+    and a loop of one aclblasCgemm3m call per problem, both as the plain call (which, like the
+    grouped members, turns the rounding-correction screen on for k >= 1024) and with the
+    correction off. Each is timed in steady state: warm-up calls first (the GPU's clock ramps
+    up over the first tens of milliseconds of load), then the timed ones. Complex TFLOP/s
+    counts sum(8 m n k)."""
     probs, keep, flops = grouped_problems(torch, cgemm3m, dev, count, seed)
     stream = torch.cuda.current_stream()
 
-    def timed(fn):
-        fn()
+    def timed(fn, warm, reps):
+        for _ in range(warm):
+            fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -356,10 +359,10 @@ def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
             cgemm3m.cgemm3m(h, p["ta"], p["tb"], p["m"], p["n"], p["k"], 1.0, p["A"], p["lda"], p["B"], p["ldb"],
                             0.0, p["C"], p["ldc"], cfg)
 
-    t_group = timed(lambda: cgemm3m.grouped(h, probs))
-    t_loop = timed(loop)
+    t_group = timed(lambda: cgemm3m.grouped(h, probs), 30, 20)
+    t_loop = timed(loop, 3, 5)
     no_corr = cgemm3m.config(correction=-1)
-    t_loop_nc = timed(lambda: loop(no_corr))
+    t_loop_nc = timed(lambda: loop(no_corr), 3, 5)
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     with torch.cuda.stream(side):
@@ -370,7 +373,7 @@ def grouped_bench(h, torch, cgemm3m, dev, count=512, seed=4242, reps=5):
         h.set_stream(torch.cuda.current_stream())
         cgemm3m.grouped(h, probs)
     h.set_stream(stream)
-    t_graph = timed(graph.replay)
+    t_graph = timed(graph.replay, 40, 30)
     return {"problems": count, "complex_tflop": flops / 1e12, "ms_grouped": t_group, "ms_graph_replay": t_graph,
             "ms_per_problem_loop": t_loop, "ms_per_problem_loop_no_correction": t_loop_nc,
             "tflops_grouped": flops / (t_group * 1e-3) / 1e12,
