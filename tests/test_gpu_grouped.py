@@ -400,3 +400,42 @@ def test_grouped_raw_reads_match_packed_planes(gpu, pin):
         assert r.tobytes() == p.tobytes(), "member %d (%s%s %d x %d x %d) differs" % (
             i, probs[i]["ta"], probs[i]["tb"], probs[i]["m"], probs[i]["n"], probs[i]["k"])
     check_all(probs, host, "grouped-raw")
+
+
+def test_grouped_edge_fragments_match_full_blocks(gpu):
+    """Blocks at a member's last block row or column with at most 32 outputs left run on one
+    32-wide fragment (32 x 64, 64 x 32 and 32 x 32 wave shapes); the packed form (variant 7)
+    computes whole 64 x 64 blocks. Both must give the same bytes: sizes on either side of each
+    32 and 64 boundary, odd sizes, k tails, all nine op pairs, the correction on and off,
+    general scalars; then the oracle's verdict per member."""
+    import torch
+
+    import cgemm3m
+
+    sizes = [(1, 1, 7), (31, 33, 100), (32, 64, 64), (33, 65, 129), (63, 31, 1100), (64, 96, 1024),
+             (65, 97, 1500), (96, 32, 250), (97, 161, 2049), (160, 33, 1030), (200, 224, 300), (17, 250, 1111)]
+    rng = np.random.default_rng(77)
+    probs, host = [], []
+    for i, (m, n, k) in enumerate(sizes):
+        ta, tb = "NTC"[i % 3], "NTC"[(i // 3) % 3]
+        lda = (m if ta == "N" else k) + int(rng.integers(0, 3))
+        ldb = (k if tb == "N" else n) + int(rng.integers(0, 3))
+        ldc = m + 1
+        A, B, C = inputs(ta, tb, m, n, k, lda, ldb, ldc, 5000 + i)
+        dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+        probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=complex(0.7, -0.3), A=ptr(dA), lda=lda, B=ptr(dB),
+                          ldb=ldb, beta=complex(0.2, 0.9), C=ptr(dC), ldc=ldc))
+        host.append(dict(A=A, B=B, C=C, dA=dA, dB=dB, dC=dC))
+    for correction in (0, -1):
+        outs = []
+        for variant in (0, 7):
+            for h in host:
+                h["dC"].copy_(torch.from_numpy(h["C"]).cuda())
+            # tile 64 pinned: every member on the persistent kernel (none on the one-launch one)
+            cgemm3m.grouped(gpu, probs, cfg=cgemm3m.config(variant=variant, correction=correction, tile=64))
+            torch.cuda.synchronize()
+            outs.append([h["dC"].cpu().numpy().copy() for h in host])
+        for i, (r, p) in enumerate(zip(*outs)):
+            same = r.tobytes() == p.tobytes()
+            assert same, ("member differs", correction, i, probs[i]["ta"], probs[i]["tb"], sizes[i])
+        check_all(probs, host, "edge-fragments correction=%d" % correction)
