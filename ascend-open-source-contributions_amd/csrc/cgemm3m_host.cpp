@@ -184,12 +184,17 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         tile = cfg->tile == 16 ? 64 : cfg->tile;
     } else {
         // The tuned table holds FP32-product measurements; other formats use the occupancy rule.
-        // (An entry naming the one-launch kernel, tile 32 or 16, is used further down.)
+        // (An entry naming the one-launch kernel, tile 32 or 16, is used further down.) The
+        // 16-bit fused kernels keep 128 x 128 tiles (split in k below) down to a quarter of
+        // the CUs' worth of tiles: FP16 1024 x 1024 x 4096 0.168 ms on 64 x 64 tiles against
+        // 0.112 ms on 128 x 128 tiles over four k partitions.
         int tt = 0, tp = 0;
         if (pl.format == FMT_F32 && tuned_lookup(m, n, k, false, &tt, &tp) && tt >= 64)
             tile = tt;
-        else
+        else if (pl.format == FMT_F32)
             tile = tiles_for(128) >= cus ? 128 : 64;
+        else
+            tile = tiles_for(128) >= cus / 4 ? 128 : 64;
     }
     pl.bm = pl.bn = tile;
     pl.mPad = (int)round_up((size_t)m, 128);
@@ -210,14 +215,23 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.parts = std::max(1, std::min(tp, pl.kSteps));
     } else {
         // Too few output tiles to fill the CUs: split k, keeping >= 256 k per partition. FP32:
-        // one workgroup per CU, at most 16 partitions. 16-bit formats (the fused kernel, whose
-        // long k loops are latency-bound at one workgroup per CU): about two per CU, at most 64
-        // (256 x 256 x 65536 bf16x3: 1.57 ms at 16 partitions).
+        // at least one workgroup per CU, at most 16 partitions. 16-bit formats on 64 x 64 tiles
+        // (two workgroups fit a CU, and the long k loops are latency-bound at one): about two
+        // per CU, at most 64 partitions (256 x 256 x 65536 bf16x3: 1.57 ms at 16 partitions).
+        // 16-bit formats on 128 x 128 tiles (one workgroup per CU): the largest split that still
+        // runs in one round, since a few workgroups more than the CUs make a second, nearly
+        // empty round. FP16 on MI355X, ms per call: 1024 x 1024 x 16384 0.434 at 4 partitions
+        // (256 workgroups) against 0.447 at 8 (512); 768 x 768 x 8192 0.132 at 7 (252) against
+        // 0.159 at 15 (540).
         pl.parts = 1;
-        const long long perCu = pl.format == FMT_F32 ? 1 : 2, cap = pl.format == FMT_F32 ? 16 : 64;
         if (tiles < cus) {
-            long long parts = (perCu * cus + tiles - 1) / tiles;
-            parts = std::min<long long>(parts, cap);
+            long long parts;
+            if (pl.format == FMT_F32)
+                parts = std::min<long long>((cus + tiles - 1) / tiles, 16);
+            else if (tile == 64)
+                parts = std::min<long long>((2 * cus + tiles - 1) / tiles, 64);
+            else
+                parts = std::min<long long>(cus / tiles, 16);
             parts = std::min<long long>(parts, std::max(1, pl.kSteps * pl.bk / 256));
             pl.parts = (int)std::max<long long>(parts, 1);
         }
@@ -244,14 +258,16 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
     // 577, bf16x6 232 vs 319; 2048^3: 291 vs 323, 173 vs 198).
     //
     // BF16 products with too few 256 x 256 tiles for the CUs split k instead (tall-skinny and
-    // small outputs): about one workgroup per CU, each partition at least 256 deep, at most
-    // 128 partitions; merge_kernel sums each product's partitions in index order first.
+    // small outputs): the most partitions that keep the workgroups within one per CU (a
+    // partial second round costs a whole round: bf16x3 1024 x 2048 x 4096 0.280 ms at 3
+    // partitions, 288 workgroups, against 0.229 ms at 2, 192), each partition at least 256
+    // deep, at most 128 partitions; merge_kernel sums each product's partitions in index
+    // order first.
     const bool pinned = cfg && (cfg->tile != 0 || cfg->kSplit != 0 || cfg->variant != 0);
     const long long tiles256 = ((m + 255) / 256) * ((n + 255) / 256);
     long long splitParts = 1;
     if (pl.format == FMT_BF16 && !pinned && 3 * tiles256 < cus / 2)
-        splitParts = std::max<long long>(
-            1, std::min<long long>({(cus + 3 * tiles256 - 1) / (3 * tiles256), 128, pl.kSteps * pl.bk / 256}));
+        splitParts = std::max<long long>(1, std::min<long long>({cus / (3 * tiles256), 128, pl.kSteps * pl.bk / 256}));
     //
     // With few outputs (at most 16 tiles of 128 x 128) the BF16 products read the operands as
     // stored instead (variant 7 pins it): no pack launch, the plane values and BF16 pieces
@@ -279,8 +295,7 @@ bool make_plan(const Problem& p, const aclblasCgemm3mConfig* cfg, int cus, Plan&
         pl.raw = raw;
         pl.bm = pl.bn = raw ? 128 : 256;
         if (raw && !pinnedSplit)
-            splitParts = std::max<long long>(
-                1, std::min<long long>({(cus + tiles128 - 1) / tiles128, 128, pl.kSteps * pl.bk / 256}));
+            splitParts = std::max<long long>(1, std::min<long long>({cus / tiles128, 128, pl.kSteps * pl.bk / 256}));
         pl.parts = pinnedSplit ? std::min(cfg->kSplit, pl.kSteps) : (int)splitParts;
         pl.mPad = (int)round_up((size_t)m, raw ? 128 : 256);
         pl.nPad = (int)round_up((size_t)n, raw ? 128 : 256);
