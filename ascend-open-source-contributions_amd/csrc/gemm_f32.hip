@@ -458,10 +458,14 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
 
     // Plane values of one element with the conjugation sign folded in: s * im is exact (s = +-1),
     // so fma(s, im, x) rounds exactly as x + (s * im) does, and the values equal
-    // plane_value(role, ID, q, re, s * im), the pack kernel's.
+    // plane_value(role, ID, q, re, s * im), the pack kernel's. The fused identity's third A
+    // plane is s * Ai; it is used here as Ai, and that product's accumulators are negated after
+    // the k loop when s = -1: rounding is symmetric in sign, so every partial sum of Ai * (Br+Bi)
+    // is exactly the negation of the one of (-Ai) * (Br+Bi), and the bytes are the pack path's
+    // with one multiply per element and fragment fewer in the loop.
     auto planeA = [&](int q, float re, float im) {
         if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? aSign * im : fmaf(aSign, im, re));
-        return q == 0 ? fmaf(aSign, im, re) : (q == 1 ? re : aSign * im);  // (Ar+Ai, Ar, Ai)
+        return q == 0 ? fmaf(aSign, im, re) : (q == 1 ? re : im);  // (Ar+Ai, Ar, Ai), Ai unsigned
     };
     auto planeB = [&](int q, float re, float im) {
         if (ID == ID_STANDARD) return q == 0 ? re : (q == 1 ? bSign * im : fmaf(bSign, im, re));
@@ -575,6 +579,14 @@ __device__ __forceinline__ void f32_tile_raw64(const GemmArgs& g, int part, int 
         mfma(Tl, t, On());
     }
 
+    if (ID != ID_STANDARD && g.src.opA == 113) {  // the third product was accumulated with Ai for s * Ai
+#pragma unroll
+        for (int a = 0; a < TN; ++a)
+#pragma unroll
+            for (int b = 0; b < TM; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[2][a][b][r] = -acc[2][a][b][r];
+    }
     const int iBase = i0 + (lane & 31), jBase = j0 + 4 * half;
     if (g.parts > 1) {  // K split: the merge screens, from energies computed before this kernel
         store_tile<TN, TM, true, 3>(g, acc, part, iBase, jBase);
