@@ -16,6 +16,8 @@
 //                    alpha/beta applied in FP64 and the result rounded once to FP32.
 //   residual_kernel  FP64 complex GEMM of the same inputs compared with the FP32 result
 //                    (error report for tests and benches only).
+#include <algorithm>
+
 #include "device_util.h"
 #include "launch.h"
 
@@ -214,13 +216,12 @@ hipError_t launch_scale_c(const ScaleCArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-__device__ __forceinline__ double2 load_op(const float2* X, long long ld, int op, long long row, long long col,
-                                           bool rowsContig)
+// op(X)(row, col): op N reads X(row, col); op T/C reads X(col, row), C conjugated (the sign
+// flip is exact, and so is the widening to FP64 at the point of use).
+__device__ __forceinline__ float2 load_op(const float2* X, long long ld, int op, long long row, long long col)
 {
-    // op(X)(row, col): op N reads X(row, col); op T/C reads X(col, row), C conjugated.
     const float2 v = (op == 111) ? X[(size_t)col * ld + row] : X[(size_t)row * ld + col];
-    (void)rowsContig;
-    return make_double2((double)v.x, op == 113 ? -(double)v.y : (double)v.y);
+    return make_float2(v.x, op == 113 ? -v.y : v.y);
 }
 
 // The flagged outputs of bitmap word w (bits = its value), recomputed by the whole workgroup.
@@ -247,34 +248,28 @@ __device__ __forceinline__ void recompute_word(const RecomputeArgs& a, long long
             bits &= bits - 1ull;
             const long long lin = w * 64 + b;
             const long long i = lin % a.m, j = lin / a.m;
-            // thread t sums k = t + 256 u in increasing u (the loads of sixteen u in flight
-            // together, the adds in u order); each wave's 64 sums by a fixed butterfly; the
-            // four wave sums added in wave order
+            // thread t sums k = t + 256 u in increasing u (the loads of 32 u in flight together,
+            // kept in FP32 until used; the adds in u order); each wave's 64 sums by a fixed
+            // butterfly; the four wave sums added in wave order
             double sr = 0.0, si = 0.0;
+            auto acc = [&](float2 x, float2 y) {
+                sr = fma((double)x.x, (double)y.x, sr);
+                sr = fma(-(double)x.y, (double)y.y, sr);
+                si = fma((double)x.x, (double)y.y, si);
+                si = fma((double)x.y, (double)y.x, si);
+            };
             long long l = tid;
-            for (; l + 256 * 15 < a.k; l += 256 * 16) {
-                double2 x[16], y[16];
+            for (; l + 256 * 31 < a.k; l += 256 * 32) {
+                float2 x[32], y[32];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    x[u] = load_op(A, a.lda, a.opA, i, l + 256 * u, false);   // op(A)(i, l)
-                    y[u] = load_op(B, a.ldb, a.opB, l + 256 * u, j, false);   // op(B)(l, j)
+                for (int u = 0; u < 32; ++u) {
+                    x[u] = load_op(A, a.lda, a.opA, i, l + 256 * u);   // op(A)(i, l)
+                    y[u] = load_op(B, a.ldb, a.opB, l + 256 * u, j);   // op(B)(l, j)
                 }
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    sr = fma(x[u].x, y[u].x, sr);
-                    sr = fma(-x[u].y, y[u].y, sr);
-                    si = fma(x[u].x, y[u].y, si);
-                    si = fma(x[u].y, y[u].x, si);
-                }
+                for (int u = 0; u < 32; ++u) acc(x[u], y[u]);
             }
-            for (; l < a.k; l += 256) {
-                const double2 x = load_op(A, a.lda, a.opA, i, l, false);
-                const double2 y = load_op(B, a.ldb, a.opB, l, j, false);
-                sr = fma(x.x, y.x, sr);
-                sr = fma(-x.y, y.y, sr);
-                si = fma(x.x, y.y, si);
-                si = fma(x.y, y.x, si);
-            }
+            for (; l < a.k; l += 256) acc(load_op(A, a.lda, a.opA, i, l), load_op(B, a.ldb, a.opB, l, j));
 #pragma unroll
             for (int d = 32; d >= 1; d >>= 1) {
                 sr += __shfl_xor(sr, d, 64);
@@ -303,7 +298,8 @@ __device__ __forceinline__ void recompute_word(const RecomputeArgs& a, long long
     }
 }
 
-__device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
+// span: bitmap words per workgroup and step, 1..256 (from recompute_span).
+__device__ __forceinline__ void recompute_body(const RecomputeArgs& a, int span)
 {
     __shared__ double part[4][2];
     __shared__ unsigned long long waveMask[4], wordVal[256];
@@ -311,12 +307,13 @@ __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
     const float2* A = reinterpret_cast<const float2*>(a.A);
     const float2* B = reinterpret_cast<const float2*>(a.B);
     float2* C = reinterpret_cast<float2*>(a.C);
-    // Each workgroup scans 256 consecutive bitmap words at a time (one coalesced load, a ballot
-    // per wave) and handles the set bits of the nonzero ones in increasing order, all 256
-    // threads on one output (flagged outputs are rare, so they spread over the workgroups).
-    for (long long base = (long long)blockIdx.x * 256; base < a.words; base += (long long)gridDim.x * 256) {
+    // Each workgroup scans `span` consecutive bitmap words at a time (one coalesced load, a
+    // ballot per wave) and handles the set bits of the nonzero ones in increasing order, all
+    // 256 threads on one output. Flagged outputs are rare: the span is chosen so that they
+    // spread over many workgroups, since one workgroup's outputs run one after another.
+    for (long long base = (long long)blockIdx.x * span; base < a.words; base += (long long)gridDim.x * span) {
         const long long wi = base + tid;
-        const unsigned long long mine = wi < a.words ? a.bitmap[wi] : 0ull;
+        const unsigned long long mine = (tid < span && wi < a.words) ? a.bitmap[wi] : 0ull;
         const unsigned long long live = __ballot(mine != 0ull);
         wordVal[tid] = mine;
         if (lane == 0) waveMask[wave] = live;
@@ -335,31 +332,45 @@ __device__ __forceinline__ void recompute_body(const RecomputeArgs& a)
 }
 
 
-__global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a) { recompute_body(a); }
+__global__ __launch_bounds__(256) void recompute_kernel(RecomputeArgs a, int span) { recompute_body(a, span); }
 
 // Grouped batch: blockIdx.y picks the member (one with the rounding correction on).
-__global__ __launch_bounds__(256) void recompute_grouped_kernel(const RecomputeArgs* items)
+__global__ __launch_bounds__(256) void recompute_grouped_kernel(const RecomputeArgs* items, int span)
 {
     const RecomputeArgs a = items[blockIdx.y];
-    recompute_body(a);
+    recompute_body(a, span);
+}
+
+// Bitmap words per workgroup and step: about `target` workgroups over `words` words, at most
+// 256 words (one per thread) and at least 1. With few words (tall-skinny outputs with long k,
+// where each flagged output is a long FP64 dot product) every word gets its own workgroup: at
+// 256 x 256 x 65536 the 1024 words took 4 workgroups, each recomputing its flagged outputs one
+// after another.
+static int recompute_span(long long words, long long target)
+{
+    long long span = (words + target - 1) / target;
+    return (int)std::min<long long>(256, std::max<long long>(1, span));
 }
 
 hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s)
 {
-    long long bx = (maxWords + 255) / 256;  // 256 bitmap words per workgroup and step
+    if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
+    // about 4096 workgroups over all members
+    const int span = recompute_span(maxWords, std::max<long long>(1, 4096 / nitems));
+    long long bx = (maxWords + span - 1) / span;
     if (bx > 4096) bx = 4096;
     if (bx < 1) bx = 1;
-    if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(recompute_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items);
+    hipLaunchKernelGGL(recompute_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items, span);
     return hipGetLastError();
 }
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)
 {
-    long long blocks = (a.words + 255) / 256;  // 256 bitmap words per workgroup and step
+    const int span = recompute_span(a.words, 2048);
+    long long blocks = (a.words + span - 1) / span;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(recompute_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, span);
     return hipGetLastError();
 }
 
@@ -387,13 +398,19 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a)
             if (a.opA == 111) { rr = e & 63; ll = e >> 6; } else { ll = e & 15; rr = e >> 4; }
             const long long gi = i0 + rr, gl = l0 + ll;
             double2 v = make_double2(0.0, 0.0);
-            if (gi < a.m && gl < a.k) v = load_op(A, a.lda, a.opA, gi, gl, false);
+            if (gi < a.m && gl < a.k) {
+                const float2 f = load_op(A, a.lda, a.opA, gi, gl);
+                v = make_double2((double)f.x, (double)f.y);
+            }
             As[ll][rr] = v;
             int cc;
             if (a.opB == 111) { ll = e & 15; cc = e >> 4; } else { cc = e & 63; ll = e >> 6; }
             const long long gj = j0 + cc, gl2 = l0 + ll;
             double2 w = make_double2(0.0, 0.0);
-            if (gj < a.n && gl2 < a.k) w = load_op(B, a.ldb, a.opB, gl2, gj, false);
+            if (gj < a.n && gl2 < a.k) {
+                const float2 f = load_op(B, a.ldb, a.opB, gl2, gj);
+                w = make_double2((double)f.x, (double)f.y);
+            }
             Bs[ll][cc] = w;
         }
         __syncthreads();
