@@ -246,7 +246,9 @@ static void raw_member_workspace(Plan& pl)
 // tile/kSplit/identity/correction override when given (variant 5: the one-launch kernel). The one-problem
 // form of the same plan (variant: LDS-free kernel for 128 tiles, as the grouped kernel uses)
 // gives byte-identical results, which the strided-batched entry relies on.
-aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over)
+bool grouped_batch_fills(long long blocks64, int cus) { return cus > 0 && blocks64 >= 8LL * cus; }
+
+aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over, bool batchFills)
 {
     aclblasCgemm3mConfig cfg{ACLBLAS_CGEMM3M_FP32, 0, 1, 0, 0, 1};
     int tt = 0, tp = 0;
@@ -272,15 +274,17 @@ aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCge
         cfg.kSplit = 1;
     } else {
         cfg.variant = cfg.tile == 128 ? 3 : 1;
-        // A member with the rounding correction on takes no K split unless the call pins one:
-        // the persistent kernel then forms its energies from its own loads, where a split
-        // member needs an energy pass over its operands beside it (whose workgroups hold CUs
-        // the persistent kernel waits for), a partial-plane merge and a longer recompute
-        // list. The persistent queue balances a batch without the split. (One rule for every
-        // form, so a member's bytes do not depend on the form.)
+        // In a batch that fills the GPU without K splits, a member with the rounding
+        // correction on takes none unless the call pins one: the persistent kernel then forms
+        // its energies from its own loads, where a split member needs an energy pass over its
+        // operands beside it (whose workgroups hold CUs the persistent kernel waits for), a
+        // partial-plane merge and a longer recompute list, and the persistent queue balances
+        // such a batch without the split. A small batch keeps the table's split, which it
+        // needs to fill the CUs. (The same rule for every form of a call, so a member's bytes
+        // do not depend on the form.)
         const bool corrected = cfg.correction != 0 ? cfg.correction > 0 : k >= 1024;
         const bool pinned = over != nullptr && (over->tile != 0 || over->kSplit != 0);
-        if (corrected && !pinned) cfg.kSplit = 1;
+        if (batchFills && corrected && !pinned) cfg.kSplit = 1;
     }
     return cfg;
 }
@@ -290,8 +294,15 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
                                     const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
                                     const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
                                     aclblasComplex* const* C, const int* ldc, bool withPointers,
-                                    const aclblasCgemm3mConfig* over, GroupedLayout& L, const char* op)
+                                    const aclblasCgemm3mConfig* over, GroupedLayout& L, const char* op,
+                                    int fills = -1)
 {
+    if (fills < 0) {
+        long long blocks64 = 0;
+        for (int i = 0; i < count; ++i)
+            if (m[i] > 0 && n[i] > 0 && k[i] > 0) blocks64 += (long long)((m[i] + 63) / 64) * ((n[i] + 63) / 64);
+        fills = grouped_batch_fills(blocks64, h->cus) ? 1 : 0;
+    }
     // variant 7 keeps the earlier form: members packed into planes by one launch first; so
     // does a batch on the standard identity (the raw kernel forms the fused identity's planes)
     L.raw = !(over != nullptr && (over->variant == 7 || over->identity == ID_STANDARD));
@@ -311,7 +322,7 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
         for (int i = 0; i < count; ++i) {
             if (m[i] <= 0 || n[i] <= 0 || k[i] <= 0) continue;
             const double w = (double)m[i] * n[i] * k[i];
-            (grouped_member_config(m[i], n[i], k[i], over).variant == 5 ? smallWork : quadWork) += w;
+            (grouped_member_config(m[i], n[i], k[i], over, fills != 0).variant == 5 ? smallWork : quadWork) += w;
         }
         absorb = smallWork > 0.0 && quadWork >= smallWork;
     }
@@ -334,7 +345,7 @@ static aclblasStatus_t build_layout(aclblasHandle_t h, int count, const aclblasO
             if (!(p.beta.real == 1.0f && p.beta.imag == 0.0f)) L.scaleOnly.push_back(p);
             continue;
         }
-        aclblasCgemm3mConfig cfg = grouped_member_config(p.m, p.n, p.k, over);
+        aclblasCgemm3mConfig cfg = grouped_member_config(p.m, p.n, p.k, over, fills != 0);
         if (absorb && cfg.variant == 5) {
             cfg.tile = 64;
             cfg.kSplit = 1;
@@ -427,7 +438,7 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
                                    const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
                                    const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
                                    aclblasComplex* const* C, const int* ldc, int precision,
-                                   const aclblasCgemm3mConfig* over, const char* op)
+                                   const aclblasCgemm3mConfig* over, const char* op, int fills)
 {
     if (handle == nullptr) return ACLBLAS_STATUS_HANDLE_IS_NULLPTR;
     if (count < 0) return ACLBLAS_STATUS_INVALID_VALUE;
@@ -448,11 +459,12 @@ aclblasStatus_t c3m::grouped_run(aclblasHandle_t handle, int count, const aclbla
         GroupedLayout L0;
         handle->cus = 0;
         const aclblasStatus_t v = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta,
-                                               C, ldc, true, over, L0, op);
+                                               C, ldc, true, over, L0, op, fills);
         return v != ACLBLAS_STATUS_SUCCESS ? v : st;
     }
     GroupedLayout L;
-    st = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, over, L, op);
+    st = build_layout(handle, count, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, over, L, op,
+                      fills);
     if (st != ACLBLAS_STATUS_SUCCESS) return st;
     hipStream_t s = handle->stream;
     hipError_t e = hipSuccess;

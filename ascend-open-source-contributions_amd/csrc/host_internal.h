@@ -167,14 +167,20 @@ aclblasStatus_t run_problem(aclblasHandle_t handle, aclblasOperation_t ta, aclbl
                             const aclblasComplex* B, int ldb, const aclblasComplex* beta, aclblasComplex* C,
                             int ldc, const aclblasCgemm3mConfig* cfg, const char* op,
                             ScratchSource scratch = nullptr);
-// One grouped batch of FP32 problems (grouped_host.cpp); over: optional plan override.
+// One grouped batch of FP32 problems (grouped_host.cpp); over: optional plan override. fills:
+// whether the batch is large enough that its corrected members take no K split (see
+// grouped_batch_fills); -1 decides from this call's members.
 aclblasStatus_t grouped_run(aclblasHandle_t handle, int count, const aclblasOperation_t* transa,
                             const aclblasOperation_t* transb, const int* m, const int* n, const int* k,
                             const aclblasComplex* alpha, const aclblasComplex* const* A, const int* lda,
                             const aclblasComplex* const* B, const int* ldb, const aclblasComplex* beta,
                             aclblasComplex* const* C, const int* ldc, int precision,
-                            const aclblasCgemm3mConfig* over, const char* op);
-// The plan a grouped member of this shape gets, as a one-problem config.
-aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over);
+                            const aclblasCgemm3mConfig* over, const char* op, int fills = -1);
+// True when a batch whose members cover `blocks64` output blocks of 64 x 64 in all gives the
+// persistent kernel at least two quads (four blocks each) per CU without any K split.
+bool grouped_batch_fills(long long blocks64, int cus);
+// The plan a grouped member of this shape gets, as a one-problem config (batchFills: the
+// batch's grouped_batch_fills).
+aclblasCgemm3mConfig grouped_member_config(int m, int n, int k, const aclblasCgemm3mConfig* over, bool batchFills);
 
 }  // namespace c3m

@@ -67,7 +67,13 @@ extern "C" aclblasStatus_t aclblasCgemm3mStridedBatched(aclblasHandle_t handle, 
     };
     const long long extent = (long long)(n - 1) * ldc + m;  // elements one output spans
     const long long gap = strideC < 0 ? -(long long)strideC : (long long)strideC;
-    const aclblasCgemm3mConfig cfg = grouped_member_config(m, n, k, nullptr);
+    // One plan for every batch of the call, whichever form runs it: the K-split rule of the
+    // grouped members is decided from the whole call's batches (and the device's CUs; the
+    // launches below report a missing device).
+    if (ensure_device(handle) != ACLBLAS_STATUS_SUCCESS) handle->cus = 0;
+    const int fills =
+        grouped_batch_fills((long long)batchCount * ((m + 63) / 64) * ((n + 63) / 64), handle->cus) ? 1 : 0;
+    const aclblasCgemm3mConfig cfg = grouped_member_config(m, n, k, nullptr, fills != 0);
     if (batchCount > 1 && gap < extent) {
         for (int i = 0; i < batchCount; ++i) {
             st = run_problem(handle, transa, transb, m, n, k, &al, at(A, strideA, i), lda, at(B, strideB, i), ldb, &be,
@@ -101,7 +107,7 @@ extern "C" aclblasStatus_t aclblasCgemm3mStridedBatched(aclblasHandle_t handle, 
         }
         st = grouped_run(handle, cnt, ta.data(), tb.data(), mm.data(), nn.data(), kk.data(), av.data(), ap.data(),
                          la.data(), bp.data(), lb.data(), bv.data(), cp.data(), lc.data(), ACLBLAS_CGEMM3M_FP32,
-                         nullptr, op);
+                         nullptr, op, fills);
         if (st != ACLBLAS_STATUS_SUCCESS) return st;
     }
     return ACLBLAS_STATUS_SUCCESS;

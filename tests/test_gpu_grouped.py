@@ -439,3 +439,38 @@ def test_grouped_edge_fragments_match_full_blocks(gpu):
             same = r.tobytes() == p.tobytes()
             assert same, ("member differs", correction, i, probs[i]["ta"], probs[i]["tb"], sizes[i])
         check_all(probs, host, "edge-fragments correction=%d" % correction)
+
+
+def test_grouped_k_split_rule_follows_batch_size(gpu):
+    """Corrected members (k >= 1024) drop the tuned table's K split only in a batch that fills
+    the GPU without it. Seen through the strided entry, whose overlapping outputs (strideC = 0)
+    run batch after batch with the member plan, recorded as the last plan: 2 batches of
+    160 x 144 x 2048 (9 blocks of 64 x 64 each) keep the table's partitions, 700 (6300 blocks)
+    run unsplit. A small grouped batch of that shape still meets the oracle."""
+    import torch
+
+    import cgemm3m
+
+    m, n, k = 160, 144, 2048
+    A = to_dev(inputs("N", "T", m, n, k, m, n, m, 4040)[0])
+    B = to_dev(inputs("N", "T", m, n, k, m, n, m, 4041)[1])
+    parts = {}
+    for batch in (2, 700):
+        C = torch.zeros(m * n, dtype=torch.complex64, device="cuda")
+        cgemm3m.strided_batched(gpu, "N", "T", m, n, k, 1.0, ptr(A), m, 0, ptr(B), n, 0, 0.0, ptr(C), m, 0, batch)
+        torch.cuda.synchronize()
+        plan = gpu.last_plan()
+        assert plan["correction"] == 1, plan
+        parts[batch] = plan["parts"]
+    assert parts[2] > 1 and parts[700] == 1, parts
+
+    probs, host = [], []
+    for i in range(2):
+        A, B, C = inputs("N", "T", m, n, k, m, n, m, 4050 + i)
+        dA, dB, dC = to_dev(A), to_dev(B), to_dev(C)
+        probs.append(dict(ta="N", tb="T", m=m, n=n, k=k, alpha=complex(0.7, -0.3), A=ptr(dA), lda=m, B=ptr(dB),
+                          ldb=n, beta=complex(0.2, 0.9), C=ptr(dC), ldc=m))
+        host.append(dict(A=A, B=B, C=C, dA=dA, dB=dB, dC=dC))
+    cgemm3m.grouped(gpu, probs)
+    torch.cuda.synchronize()
+    check_all(probs, host, "small batch, K split kept")
