@@ -178,9 +178,11 @@ aclblasStatus_t aclblasCgemm3mGetWorkspaceSize(aclblasHandle_t handle, aclblasOp
 /* For callers with many small independent problems: `count` problems in one persistent
  * launch that reads their operands as stored (plus, if any problem has the rounding
  * correction on, one energy launch before it and one recompute launch after it, and if any
- * is K-split, one merge launch);
- * problems the autotuned table gives to the one-launch kernel run in one more launch per
- * form, on a second stream of the handle joined back before the call returns.
+ * is K-split, one merge launch). In a batch large enough to fill the GPU without K splits,
+ * members with the correction on take none unless the call pins one; problems the autotuned
+ * table gives to the one-launch kernel join the persistent launch when the batch's other
+ * problems carry at least as much work, and otherwise run in one more launch per form, on
+ * a second stream of the handle joined back before the call returns.
  * All arrays are HOST arrays of length count; A/B/C entries are device pointers.
  * Per-problem validation follows aclblasCgemm3m; the first invalid problem's status is
  * returned and nothing is launched. precision is an aclblasCgemm3mPrecision_t. */
