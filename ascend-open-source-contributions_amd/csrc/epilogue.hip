@@ -168,13 +168,23 @@ __global__ __launch_bounds__(256) void merge_grouped_kernel(const GroupMerge* it
         merge_one(a, (int)(t % a.m), (int)(t / a.m));
 }
 
+// Grouped launches put the member on grid y, which holds at most 65535: larger batches take
+// one launch per 65535 members (the members are independent).
+constexpr int kMaxGridY = 65535;
+
 hipError_t launch_merge_grouped(const GroupMerge* items, int nitems, int maxOutputs, hipStream_t s)
 {
     int bx = (maxOutputs + 255) / 256;
     if (bx > 1024) bx = 1024;
     if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(merge_grouped_kernel, dim3(bx, nitems), dim3(256), 0, s, items);
-    return hipGetLastError();
+    if (nitems <= 0) return hipErrorInvalidValue;
+    for (int base = 0; base < nitems; base += kMaxGridY) {
+        const int ny = std::min(kMaxGridY, nitems - base);
+        hipLaunchKernelGGL(merge_grouped_kernel, dim3(bx, ny), dim3(256), 0, s, items + base);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 static unsigned grid_y(int n) { return (unsigned)(n < 65535 ? n : 65535); }
@@ -354,14 +364,20 @@ static int recompute_span(long long words, long long target)
 
 hipError_t launch_recompute_grouped(const RecomputeArgs* items, int nitems, long long maxWords, hipStream_t s)
 {
-    if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
+    if (nitems <= 0) return hipErrorInvalidValue;
     // about 4096 workgroups over all members
     const int span = recompute_span(maxWords, std::max<long long>(1, 4096 / nitems));
     long long bx = (maxWords + span - 1) / span;
     if (bx > 4096) bx = 4096;
     if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(recompute_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items, span);
-    return hipGetLastError();
+    for (int base = 0; base < nitems; base += kMaxGridY) {
+        const int ny = std::min(kMaxGridY, nitems - base);
+        hipLaunchKernelGGL(recompute_grouped_kernel, dim3((unsigned)bx, (unsigned)ny), dim3(256), 0, s, items + base,
+                           span);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_recompute(const RecomputeArgs& a, hipStream_t s)

@@ -514,9 +514,15 @@ hipError_t launch_energy_grouped(const EnergyArgs* items, int nitems, long long 
     long long bx = (maxWork + 255) / 256;
     if (bx > 1024) bx = 1024;
     if (bx < 1) bx = 1;
-    if (nitems <= 0 || nitems > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(energy_grouped_kernel, dim3((unsigned)bx, (unsigned)nitems), dim3(256), 0, s, items);
-    return hipGetLastError();
+    if (nitems <= 0) return hipErrorInvalidValue;
+    // the member is on grid y (at most 65535): larger batches take one launch per 65535
+    for (int base = 0; base < nitems; base += 65535) {
+        const int ny = nitems - base < 65535 ? nitems - base : 65535;
+        hipLaunchKernelGGL(energy_grouped_kernel, dim3((unsigned)bx, (unsigned)ny), dim3(256), 0, s, items + base);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 long long energy_work(int mnPadA, int mnPadB, int kTiles, long long bitmapWords)

@@ -474,3 +474,30 @@ def test_grouped_k_split_rule_follows_batch_size(gpu):
     cgemm3m.grouped(gpu, probs)
     torch.cuda.synchronize()
     check_all(probs, host, "small batch, K split kept")
+
+
+def test_grouped_more_members_than_grid_y(gpu):
+    """A batch with more corrected members than a launch's grid y holds (65535): 70000 members
+    of 16 x 16 x 1024 on the persistent kernel (tile 64 pinned), all reading the same A and B,
+    each writing its own C. The bitmap-clear and recompute launches take the members in chunks;
+    every member must equal the first byte for byte, and the first must meet the oracle."""
+    import torch
+
+    import cgemm3m
+
+    count, m, n, k = 70000, 16, 16, 1024
+    A, B, C = inputs("N", "N", m, n, k, m, k, m, 777)
+    dA, dB = to_dev(A), to_dev(B)
+    dC = torch.zeros(count * m * n, dtype=torch.complex64, device="cuda")
+    base = dC.data_ptr()
+    probs = [dict(ta="N", tb="N", m=m, n=n, k=k, alpha=complex(0.7, -0.3), A=ptr(dA), lda=m, B=ptr(dB), ldb=k,
+                  beta=0.0, C=base + i * m * n * 8, ldc=m) for i in range(count)]
+    cgemm3m.grouped(gpu, probs, cfg=cgemm3m.config(tile=64, k_split=1))
+    torch.cuda.synchronize()
+    out = dC.cpu().numpy().reshape(count, m * n)
+    first = out[0].tobytes()
+    same = all(out[i].tobytes() == first for i in range(count))
+    assert same, "members differ"
+    gold = oracle.cgemm("N", "N", m, n, k, complex(0.7, -0.3), A, m, B, k, 0.0, np.zeros(m * n, np.complex64), m)
+    ok, stats = oracle.verify(out[0], gold, np.zeros(m * n, np.complex64), m, n, m)
+    assert ok, stats
