@@ -24,17 +24,36 @@
 
 namespace c3m {
 
+static const TunedEntry* tuned_bin(int mnl, int kl, bool grouped)
+{
+    for (const TunedEntry& e : kTunedTable)
+        if (e.tile != 0 && e.mnLog2 == mnl && e.kLog2 == kl && e.grouped == (grouped ? 1 : 0)) return &e;
+    return nullptr;
+}
+
 bool tuned_lookup(long long m, long long n, long long k, bool grouped, int* tile, int* parts)
 {
     if (m <= 0 || n <= 0 || k <= 0) return false;
     int mnl = 0, kl = 0;
     for (unsigned long long v = (unsigned long long)(m * n); v > 1; v >>= 1) ++mnl;
     for (unsigned long long v = (unsigned long long)k; v > 1; v >>= 1) ++kl;
-    for (const TunedEntry& e : kTunedTable) {
-        if (e.tile == 0) continue;
-        if (e.mnLog2 == mnl && e.kLog2 == kl && e.grouped == (grouped ? 1 : 0)) {
-            *tile = e.tile;
-            *parts = e.parts;
+    if (const TunedEntry* e = tuned_bin(mnl, kl, grouped)) {
+        *tile = e->tile;
+        *parts = e->parts;
+        return true;
+    }
+    // A one-problem class the tuner did not sample, between two k classes of the same output
+    // size that both take the one-launch kernel without k slices, takes it too (single-wave
+    // form when both do, else four waves). On 47 seeded random shapes in the 11 such classes
+    // (outputs up to 2^21, k to 2^8) it ran 1.35x faster (geometric mean of kernel-time sums)
+    // than the occupancy rule's packed plan, faster on 39, at most 8% slower on the others.
+    if (!grouped) {
+        const TunedEntry* lo = tuned_bin(mnl, kl - 1, false);
+        const TunedEntry* hi = tuned_bin(mnl, kl + 1, false);
+        const auto oneLaunch = [](const TunedEntry* e) { return e != nullptr && (e->tile == 16 || e->tile == 32) && e->parts == 1; };
+        if (oneLaunch(lo) && oneLaunch(hi)) {
+            *tile = (lo->tile == 16 && hi->tile == 16) ? 16 : 32;
+            *parts = 1;
             return true;
         }
     }
