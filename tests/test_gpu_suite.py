@@ -4,6 +4,7 @@ untouched), and a byte-identical second run, as the reference's accuracy suite j
 (ops-blas/test/gemm/cgemm3m/arch35/cgemm3m_test.cpp:282-310)."""
 import sys
 
+import numpy as np
 import pytest
 
 import oracle
@@ -74,3 +75,58 @@ def test_generated_sweep_grouped(gpu, call):
     for i, (h, out) in enumerate(zip(host, first)):
         same = h["dC"].cpu().numpy().tobytes() == out.tobytes()
         assert same, "member %d: second call differs" % i
+
+
+STRIDED = SWEEP[::16]
+
+
+def _buffer(count, seed):
+    return oracle.uniform(seed, 0, 2 * max(count, 1)).view(np.complex64).copy()
+
+
+@pytest.mark.parametrize("part", range(2))
+def test_generated_sweep_strided(gpu, part):
+    """Every 16th sweep case through aclblasCgemm3mStridedBatched with 1 to 4 batches: input
+    strides of zero (shared operand) or the stored extent plus a gap, output strides past the
+    output's ldc x n span (disjoint outputs). Each batch against the oracle with the verifier; the
+    gaps between outputs untouched; a second call byte-identical."""
+    import torch
+
+    import cgemm3m
+
+    rng = np.random.default_rng(20261205 + part)
+    for idx, c in enumerate(STRIDED[part::2]):
+        ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+        lda, ldb, ldc = c["lda"], c["ldb"], c["ldc"]
+        batch = 1 + (idx % 4)
+        ext_a = lda * ((k if ta == "N" else m) - 1) + (m if ta == "N" else k)
+        ext_b = ldb * ((n if tb == "N" else k) - 1) + (k if tb == "N" else n)
+        span_c = ldc * n  # the output with its padding rows
+        sa = 0 if rng.random() < 0.25 else ext_a + int(rng.integers(0, 9))
+        sb = 0 if rng.random() < 0.25 else ext_b + int(rng.integers(0, 9))
+        sc = span_c + int(rng.integers(0, 9))
+        label = "strided %s%s %dx%dx%d batch=%d strides=(%d,%d,%d) alpha=%s beta=%s" % (
+            ta, tb, m, n, k, batch, sa, sb, sc, c["alpha"], c["beta"])
+        print("case", label)
+        sys.stdout.flush()
+        Ab = _buffer(sa * (batch - 1) + ext_a, c["seed"])
+        Bb = _buffer(sb * (batch - 1) + ext_b, c["seed"] + 1)
+        Cb = _buffer(sc * (batch - 1) + span_c, c["seed"] + 2)
+        dA, dB = to_dev(Ab), to_dev(Bb)
+        outs = []
+        for _ in range(2):
+            dC = to_dev(Cb)
+            cgemm3m.strided_batched(gpu, ta, tb, m, n, k, c["alpha"], ptr(dA), lda, sa, ptr(dB), ldb, sb,
+                                    c["beta"], ptr(dC), ldc, sc, batch)
+            torch.cuda.synchronize()
+            outs.append(dC.cpu().numpy())
+        assert outs[0].tobytes() == outs[1].tobytes(), "%s: second call differs" % label
+        out = outs[0]
+        touched = np.zeros(Cb.size, dtype=bool)
+        for i in range(batch):
+            a, b, c0 = Ab[i * sa: i * sa + ext_a], Bb[i * sb: i * sb + ext_b], Cb[i * sc: i * sc + span_c]
+            gold = oracle.cgemm(ta, tb, m, n, k, c["alpha"], a, lda, b, ldb, c["beta"], c0, ldc)
+            check_against_oracle(out[i * sc: i * sc + span_c], gold, c0, m, n, ldc, "%s batch %d" % (label, i))
+            touched[i * sc: i * sc + span_c] = True
+        gaps = out[~touched].tobytes() == Cb[~touched].tobytes()
+        assert gaps, "%s: output gaps were written" % label
