@@ -130,3 +130,37 @@ def test_generated_sweep_strided(gpu, part):
             touched[i * sc: i * sc + span_c] = True
         gaps = out[~touched].tobytes() == Cb[~touched].tobytes()
         assert gaps, "%s: output gaps were written" % label
+
+
+# normwise error bound against FP64 of each 16-bit form (uniform [-1, 1) inputs, k <= 1024)
+FORM_BOUND = {"bf16x3": 6e-5, "bf16x6": 4e-6, "fp16": 4e-3}
+
+
+@pytest.mark.parametrize("prec", sorted(FORM_BOUND))
+def test_generated_sweep_precision_forms(gpu, prec):
+    """Every 11th sweep case on a 16-bit form: normwise error against FP64 (the GPU residual)
+    within the form's bound, nothing non-finite, a second run byte-identical."""
+    import torch
+
+    import cgemm3m
+
+    cfg = cgemm3m.config(prec)
+    for c in SWEEP[::11]:
+        ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+        label = "%s %s%s %dx%dx%d alpha=%s beta=%s" % (prec, ta, tb, m, n, k, c["alpha"], c["beta"])
+        print("case", label)
+        sys.stdout.flush()
+        A, B, C = inputs(ta, tb, m, n, k, c["lda"], c["ldb"], c["ldc"], c["seed"])
+        dA, dB = to_dev(A), to_dev(B)
+        st, out = run_gpu(gpu, ta, tb, m, n, k, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C, c["ldc"], cfg,
+                          dA, dB)
+        assert st == 0, (label, st)
+        st2, out2 = run_gpu(gpu, ta, tb, m, n, k, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C, c["ldc"], cfg,
+                            dA, dB)
+        assert st2 == 0 and out2.tobytes() == out.tobytes(), "%s: second run differs" % label
+        assert np.isfinite(out).all(), label
+        dC0, dOut = to_dev(C), to_dev(out)
+        res = cgemm3m.residual(gpu, ta, tb, m, n, k, c["alpha"], ptr(dA), c["lda"], ptr(dB), c["ldb"], c["beta"],
+                               ptr(dC0), ptr(dOut), c["ldc"])
+        torch.cuda.synchronize()
+        assert res["normwise"] <= FORM_BOUND[prec], (label, res)
