@@ -49,6 +49,9 @@ def main():
                     help="for rocprofv3 counter passes: 3 NN calls of the first --variants form (fp32 "
                          "by default), then one device copy of exactly 1 GiB (calibrates FETCH_SIZE/"
                          "WRITE_SIZE), nothing timed")
+    ap.add_argument("--profile-variant", type=int, default=0,
+                    help="with --profile-run: the kernel variant to pin (aclblasCgemm3mConfig.variant; 0 = the "
+                         "planner's choice)")
     ap.add_argument("--grouped-only", action="store_true",
                     help="for kernel traces: only the 512-problem grouped batch (one call, its hipGraph "
                          "replay, the per-problem loop); prints its figures as one JSON line. With "
@@ -106,12 +109,15 @@ def main():
         form = args.variants.split(",")[0] if args.variants != ap.get_default("variants") else "fp32"
         C = C0.clone()
         for _ in range(3):
-            call(cgemm3m.config(form), "NN", C)
+            st = call(cgemm3m.config(form, variant=args.profile_variant), "NN", C)
+            if st != 0:
+                log("profile run: %s variant %d: status %s" % (form, args.profile_variant, cgemm3m._lib.STATUS.get(st)))
+                sys.exit(1)
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        log("profile run done: 3 x %s NN %d^3, then a 1 GiB copy" % (form, m))
+        log("profile run done: 3 x %s (variant %d) NN %d^3, then a 1 GiB copy" % (form, args.profile_variant, m))
         return
 
     results = {}
