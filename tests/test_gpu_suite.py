@@ -164,3 +164,30 @@ def test_generated_sweep_precision_forms(gpu, prec):
                                ptr(dC0), ptr(dOut), c["ldc"])
         torch.cuda.synchronize()
         assert res["normwise"] <= FORM_BOUND[prec], (label, res)
+
+
+def test_generated_sweep_four_product_cgemm(gpu):
+    """Every 8th sweep case through aclblasCgemm (four real products): against the oracle with
+    the verifier, padding rows untouched, a second call byte-identical."""
+    import torch
+
+    import cgemm3m
+
+    for c in SWEEP[::8]:
+        ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+        label = "cgemm %s%s %dx%dx%d alpha=%s beta=%s" % (ta, tb, m, n, k, c["alpha"], c["beta"])
+        print("case", label)
+        sys.stdout.flush()
+        A, B, C = inputs(ta, tb, m, n, k, c["lda"], c["ldb"], c["ldc"], c["seed"])
+        dA, dB = to_dev(A), to_dev(B)
+        outs = []
+        for _ in range(2):
+            dC = to_dev(C)
+            st = cgemm3m.cgemm(gpu, ta, tb, m, n, k, c["alpha"], ptr(dA), c["lda"], ptr(dB), c["ldb"], c["beta"],
+                               ptr(dC), c["ldc"], check=False)
+            assert st == 0, (label, st)
+            torch.cuda.synchronize()
+            outs.append(dC.cpu().numpy())
+        assert outs[0].tobytes() == outs[1].tobytes(), "%s: second call differs" % label
+        gold = oracle.cgemm(ta, tb, m, n, k, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C, c["ldc"])
+        check_against_oracle(outs[0], gold, C, m, n, c["ldc"], label)
