@@ -82,3 +82,37 @@ def test_no_alloc_rejects_a_plan_beyond_the_array(gpu):
     del dA, dB, dC
     # the entry still works after a rejection
     _run_pair(gpu, SECOND)
+
+
+def test_no_alloc_generated_sweep(gpu):
+    """Every 9th case of the generated sweep (sizes 1 to 1024, every op pair, scalar sets with
+    alpha = 0, padded leading dimensions), one after another on the same array: each output
+    byte-identical to aclblasCgemm3m's on the same inputs. A case whose plan needs more than
+    the 64 MiB array must be refused with NOT_SUPPORTED and leave C alone."""
+    import torch
+
+    import cgemm3m
+    from cgemm3m import _lib
+    from suite_cases import suite_cases
+
+    served = 0
+    for c in [x for x in suite_cases() if x["cat"] == "sweep"][::9]:
+        ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+        case = (ta, tb, m, n, k, c["lda"], c["ldb"], c["ldc"], (c["alpha"].real, c["alpha"].imag),
+                (c["beta"].real, c["beta"].imag))
+        A, B, C = helpers.inputs(ta, tb, m, n, k, c["lda"], c["ldb"], c["ldc"], c["seed"])
+        dA, dB = helpers.to_dev(A), helpers.to_dev(B)
+        dRef, dOut = helpers.to_dev(C), helpers.to_dev(C)
+        need = cgemm3m.workspace_size(gpu, ta, tb, m, n, k, c["alpha"], c["beta"])
+        assert _entry(gpu, "aclblasCgemm3m", case, dA, dB, dRef) == 0
+        st = _entry(gpu, "aclblasCgemm3mNoAlloc", case, dA, dB, dOut)
+        torch.cuda.synchronize()
+        if need > (64 << 20):
+            assert st == _lib.NOT_SUPPORTED, (case, need, st)
+            assert dOut.cpu().numpy().tobytes() == C.tobytes(), "a refused call wrote C"
+            continue
+        assert st == 0, (case, need, st)
+        served += 1
+        same = dRef.cpu().numpy().tobytes() == dOut.cpu().numpy().tobytes()
+        assert same, "no-alloc output differs from aclblasCgemm3m's: %s" % (case,)
+    assert served >= 60, served
