@@ -132,6 +132,7 @@ def test_generated_sweep_strided(gpu, part):
         assert gaps, "%s: output gaps were written" % label
 
 
+ALPHA_X = complex(1.0, 0.5)
 # normwise error bound against FP64 of each 16-bit form (uniform [-1, 1) inputs, k <= 1024)
 FORM_BOUND = {"bf16x3": 6e-5, "bf16x6": 4e-6, "fp16": 4e-3}
 
@@ -191,3 +192,52 @@ def test_generated_sweep_four_product_cgemm(gpu):
         assert outs[0].tobytes() == outs[1].tobytes(), "%s: second call differs" % label
         gold = oracle.cgemm(ta, tb, m, n, k, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C, c["ldc"])
         check_against_oracle(outs[0], gold, C, m, n, c["ldc"], label)
+
+
+def test_generated_sweep_user_workspace(gpu):
+    """Every 10th sweep case on a handle given a 48 MiB user workspace (aclblasSetWorkspace):
+    the output is byte-identical to the default handle's, or, where the plan needs more than
+    the workspace holds, the call returns EXECUTION_FAILED and leaves C alone."""
+    import ctypes
+
+    import torch
+
+    import cgemm3m
+    from cgemm3m import _lib
+
+    ws_bytes = 48 << 20
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    h = cgemm3m.Handle()
+    try:
+        assert h.L.aclblasSetWorkspace(h.h, ctypes.c_void_p(ws.data_ptr()), ctypes.c_size_t(ws_bytes)) == 0
+        served = refused = 0
+        for c in SWEEP[::10]:
+            ta, tb, m, n, k = c["ta"], c["tb"], c["m"], c["n"], c["k"]
+            A, B, C = inputs(ta, tb, m, n, k, c["lda"], c["ldb"], c["ldc"], c["seed"])
+            dA, dB = to_dev(A), to_dev(B)
+            st, ref = run_gpu(gpu, ta, tb, m, n, k, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C, c["ldc"],
+                              None, dA, dB)
+            assert st == 0
+            need = cgemm3m.workspace_size(h, ta, tb, m, n, k, c["alpha"], c["beta"])
+            st, out = run_gpu(h, ta, tb, m, n, k, c["alpha"], A, c["lda"], B, c["ldb"], c["beta"], C, c["ldc"],
+                              None, dA, dB)
+            if need > ws_bytes:
+                assert st == _lib.EXECUTION_FAILED, (c, need, st)
+                assert out.tobytes() == C.tobytes(), "a refused call wrote C"
+                refused += 1
+                continue
+            assert st == 0, (c, need, st)
+            served += 1
+            assert out.tobytes() == ref.tobytes(), "user-workspace output differs: %s" % (c,)
+        # a plan beyond the workspace: 2048^3 packs 100 MB of planes
+        A, B, C = inputs("N", "N", 2048, 2048, 2048, 2048, 2048, 2048, 20261800)
+        need = cgemm3m.workspace_size(h, "N", "N", 2048, 2048, 2048, ALPHA_X, 0.0)
+        assert need > ws_bytes
+        st, out = run_gpu(h, "N", "N", 2048, 2048, 2048, ALPHA_X, A, 2048, B, 2048, 0.0, C, 2048)
+        assert st == _lib.EXECUTION_FAILED, st
+        assert out.tobytes() == C.tobytes(), "a refused call wrote C"
+        refused += 1
+        print("user workspace: %d served, %d refused" % (served, refused))
+        assert served >= 50
+    finally:
+        h.close()
